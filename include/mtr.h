@@ -1,0 +1,203 @@
+/*
+ * mtr.h -- C ABI of the MI355X multi-axis tomography reconstructor.
+ *
+ * Plain C: every argument is a POD struct, a pointer or a size; no HIP,
+ * MPI or torch types cross this boundary.  Every function returns an int
+ * status (MTR_OK == 0, a negative MTR_ERR_* otherwise) and never exits the
+ * process; mtr_last_error() describes the last failure of the calling
+ * thread.
+ *
+ * The entry points replace the reference's GPU host wrappers and the parts
+ * of its driver that form the SIRT / MACE hot path (file:line in
+ * Autumn1998/Multi-tilt-Reconstuction):
+ *   mtr_bpt            ComputeInitialModel      FunctionOnGPU.cu:138-179 (declared FunctionOnGPU.cuh:41)
+ *   mtr_sirt           runSIRTOnGPU             FunctionOnGPU.cu:182-266 (declared FunctionOnGPU.cuh:42)
+ *   mtr_form_z         initialZdata             FunctionOnGPU.cu:291-313 (declared FunctionOnGPU.cuh:43)
+ *   mtr_mann           runMaceOnCPU             MultiAxis.cpp:33-38 (GPU twin runMaceOnGPU FunctionOnGPU.cuh:44)
+ *   mtr_mean           consensus average        MultiAxis.cpp:61-66
+ *   mtr_mace_init      BPT-sum initialisation   MultiAxis.cpp:163-200
+ *   mtr_mace_solve     reconstructModel         MultiAxis.cpp:40-71 (+ the initialisation above)
+ *   mtr_op_fp          computeDivisor + Reproj  FunctionOnGPU.cu:316-333, :27-52
+ *   mtr_op_residual    host residual loop       FunctionOnGPU.cu:228-236
+ *   mtr_op_bp          sirtBackProjOnGPU        FunctionOnGPU.cu:111-135 (+ BilinearValue :54-79)
+ *   mtr_op_bpt         backProjOnGPU            FunctionOnGPU.cu:82-109
+ *   mtr_divide_slice   divideSlice              MultiAxis.cpp:13-31
+ * The volume and projection descriptors are layout-identical to the
+ * reference's VolumeSize and ProjectionSize (MrcFileIOBaseFunction.h:57-78);
+ * geometry is the reference's per-view affine coefficients
+ * xc[4*AngN], yc[4*AngN] (u = xc0 + xc1*X + xc2*Y + xc3*Z - 1,
+ * FunctionOnGPU.cu:12-25).
+ *
+ * Threading: one context per device and host thread; a context is not
+ * internally locked.  Device state (projections, geometry, volumes) stays
+ * resident in HBM for the context's lifetime; host output pointers are
+ * optional so a device-only pipeline never crosses PCIe.
+ */
+#ifndef MTR_H_
+#define MTR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MTR_ABI_VERSION 1
+
+enum {
+    MTR_OK = 0,
+    MTR_ERR_ARG = -1,       /* null pointer / bad enum / bad count            */
+    MTR_ERR_SHAPE = -2,     /* inconsistent or unsupported sizes              */
+    MTR_ERR_OOM = -3,       /* device allocation failed / does not fit        */
+    MTR_ERR_HIP = -4,       /* HIP runtime error                              */
+    MTR_ERR_NONFINITE = -5, /* NaN or Inf in an input                         */
+    MTR_ERR_STATE = -6,     /* call order (e.g. solve before agents)          */
+    MTR_ERR_RANGE = -7,     /* index out of range                             */
+    MTR_ERR_NODEVICE = -8   /* no usable GPU                                  */
+};
+
+/* Layout-identical to VolumeSize, MrcFileIOBaseFunction.h:57-70. */
+typedef struct {
+    int Xstart, Xend, Ystart, Yend, Zstart, Zend;
+    int X, Y, Z;
+} mtr_volume_size;
+
+/* Layout-identical to ProjectionSize, MrcFileIOBaseFunction.h:73-78
+ * (X = detector columns nu, Y = detector rows nv, AngN = views). */
+typedef struct {
+    int X, Y, AngN;
+} mtr_projection_size;
+
+/* Layout-identical to Slice, MrcFileIOBaseFunction.h:50-54. */
+typedef struct {
+    int z_start, z_end;
+} mtr_slice;
+
+/* Storage precision of volumes and projections.  Ray and back-projection
+ * sums always accumulate in FP32; residual and consensus norms in FP64. */
+enum { MTR_PREC_FP32 = 0, MTR_PREC_BF16 = 1, MTR_PREC_FP16 = 2 };
+
+/* Execution form of mtr_mace_solve / mtr_sirt. */
+enum {
+    MTR_FORM_FAST = 0,  /* fused, batched over agents, graph-replayed        */
+    MTR_FORM_PLAIN = 1  /* one agent at a time, one launch per phase, host
+                           sync per iteration, splat FP with float atomics
+                           and the normaliser recomputed every iteration:
+                           the reference's structure on this GPU           */
+};
+
+/* Volumes addressable by mtr_get_volume / mtr_set_volume. */
+enum {
+    MTR_VOL_WORK = 0, /* agent working volume (z before SIRT, v after)       */
+    MTR_VOL_W = 1,    /* agent MACE state w_a                                */
+    MTR_VOL_WBAR = 2  /* consensus w-bar (agent index ignored)               */
+};
+
+typedef struct mtr_ctx mtr_ctx;
+
+int mtr_abi_version(void);
+const char *mtr_last_error(void);
+
+/* ---- context ------------------------------------------------------- */
+int mtr_ctx_create(int device, mtr_ctx **out);
+int mtr_ctx_destroy(mtr_ctx *ctx);
+/* Storage precision for agents added afterwards (default FP32). */
+int mtr_ctx_set_precision(mtr_ctx *ctx, int precision);
+/* The reconstruction box shared by every agent (the union box the
+ * reference computes in synchronizeVol, MultiAxis.cpp:73-99).  Must be set
+ * before the first agent is added; allocates w-bar. */
+int mtr_ctx_set_volume(mtr_ctx *ctx, const mtr_volume_size *vol);
+
+/* ---- agents (one tilt series each) ---------------------------------- */
+/* Uploads projections p[AngN][Y][X] and geometry once; they stay resident.
+ * Allocates the agent's working volume, MACE state, residual and
+ * normaliser. */
+int mtr_agent_add(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc,
+                  const double *yc, const float *p_host, int *agent_id);
+/* Same, with p already in device memory (not copied; must stay valid
+ * until the context is destroyed).  p_dev is float for every precision
+ * and is converted into the agent's own storage. */
+int mtr_agent_add_device(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc,
+                         const double *yc, const float *p_dev, int *agent_id);
+int mtr_agent_count(const mtr_ctx *ctx, int *count);
+/* Bytes of HBM an agent of this shape occupies at the context precision. */
+int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size_t *bytes);
+
+/* ---- the reference's hot-path entry points, device-resident ---------- */
+/* BPT of one agent into its working volume (zeroed first).  out_host
+ * (nullable) receives the volume. */
+int mtr_bpt(mtr_ctx *ctx, int agent, float *out_host);
+/* iters SIRT iterations (the reference hard-codes 10) of one agent on its
+ * working volume.  x0_host (nullable) overwrites the working volume first;
+ * out_host (nullable) receives the result. */
+int mtr_sirt(mtr_ctx *ctx, int agent, const float *x0_host, int iters, float lambda,
+             float *out_host);
+/* z_a = 2*wbar - w_a into the agent's working volume. */
+int mtr_form_z(mtr_ctx *ctx, int agent);
+/* w_a = rho*(2*v_a - z_a) + (1-rho)*w_a with v_a the working volume and
+ * z_a = 2*wbar - w_a recomputed from the state the step started from. */
+int mtr_mann(mtr_ctx *ctx, int agent, float rho);
+/* wbar = mean over agents of w_a. */
+int mtr_mean(mtr_ctx *ctx);
+/* BPT of every agent, summed: w_a = wbar = sum_a BPT_a. */
+int mtr_mace_init(mtr_ctx *ctx);
+/* The whole reconstruction: mtr_mace_init then M MACE steps of
+ * S SIRT iterations per agent.  wbar_host (nullable) receives w-bar. */
+int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form,
+                   float *wbar_host);
+
+/* ---- state access ---------------------------------------------------- */
+int mtr_get_volume(mtr_ctx *ctx, int which, int agent, float *host);
+int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host);
+/* Device pointer of w-bar (FP32 storage only; for device-side consumers). */
+int mtr_wbar_device(mtr_ctx *ctx, void **dev_ptr);
+
+/* ---- convergence norms (FP64) ---------------------------------------- */
+/* sum of r^2 over the last SIRT iteration's residual of an agent. */
+int mtr_residual_norm2(mtr_ctx *ctx, int agent, double *out);
+/* sum over agents of ||w_a - wbar||^2 after the last consensus step. */
+int mtr_consensus_diff2(mtr_ctx *ctx, double *out);
+
+/* ---- stateless operators (host buffers in, host buffers out) --------- */
+/* Splat-equivalent forward projection of volume x (vol box, [Z][Y][X])
+ * into s and c ([AngN][Y][X]), computed as a deterministic ray-driven
+ * gather. */
+int mtr_op_fp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
+              const double *xc, const double *yc, const float *x, float *s, float *c);
+/* r = p - (c != 0 ? s / c : s). */
+int mtr_op_residual(mtr_ctx *ctx, int64_t n, const float *p, const float *s, const float *c,
+                    float *r);
+/* x += fl32(S/C) * lambda where C != 0 (in place on x). */
+int mtr_op_bp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
+              const double *xc, const double *yc, const float *r, float *x, float lambda);
+/* x = fl32(S/C) of p (x need not be initialised). */
+int mtr_op_bpt(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
+               const double *xc, const double *yc, const float *p, float *x);
+/* Z-slab partition of the reference (MultiAxis.cpp:13-31). */
+int mtr_divide_slice(int myid, int nparts, int Z, int Zstart, mtr_slice *out);
+
+/* ---- kernel timing (HIP events on the context's stream) -------------- */
+enum {
+    MTR_K_FP = 0,        /* forward projection + fused residual            */
+    MTR_K_BP = 1,        /* back-projection update                          */
+    MTR_K_BPT = 2,       /* initial back-projection                         */
+    MTR_K_CONSENSUS = 3, /* Mann + mean + next z (fused)                    */
+    MTR_K_NORMALISER = 4,/* geometry-only FP normaliser                     */
+    MTR_K_ELEMENTWISE = 5,/* standalone z / Mann / mean / residual          */
+    MTR_K_SPLAT = 6,     /* plain form: atomic splat FP                     */
+    MTR_K_COUNT = 7
+};
+/* Enable (1) or disable (0) per-launch event timing. */
+int mtr_timing_enable(mtr_ctx *ctx, int on);
+int mtr_timing_reset(mtr_ctx *ctx);
+/* Launch count and summed event time (ms) of one kernel class. */
+int mtr_timing_get(mtr_ctx *ctx, int kernel, int64_t *launches, double *ms);
+/* Blocks until all queued work of the context is done. */
+int mtr_sync(mtr_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTR_H_ */

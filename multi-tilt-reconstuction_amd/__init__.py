@@ -1,0 +1,274 @@
+"""MI355X-native multi-axis tomography reconstructor: Python binding.
+
+A thin ctypes layer over the C ABI in ``include/mtr.h`` (``libmtr.so``, built
+in-tree by ``csrc/Makefile``).  Nothing here computes: every operation runs
+in the HIP library, and importing the package fails loudly when the library
+has not been built.
+
+Reference correspondence (Autumn1998/Multi-tilt-Reconstuction):
+  Context.bpt            ComputeInitialModel   FunctionOnGPU.cu:138-179
+  Context.sirt           runSIRTOnGPU          FunctionOnGPU.cu:182-266
+  Context.form_z         initialZdata          FunctionOnGPU.cu:291-313
+  Context.mann           runMaceOnCPU          MultiAxis.cpp:33-38
+  Context.mean           consensus average     MultiAxis.cpp:61-66
+  Context.mace_init      BPT-sum initial model MultiAxis.cpp:163-200
+  Context.mace_solve     reconstructModel      MultiAxis.cpp:40-71
+  Context.op_fp          computeDivisor/Reproj FunctionOnGPU.cu:316-333, 27-52
+  Context.op_residual    residual loop         FunctionOnGPU.cu:228-236
+  Context.op_bp          sirtBackProjOnGPU     FunctionOnGPU.cu:111-135
+  Context.op_bpt         backProjOnGPU         FunctionOnGPU.cu:82-109
+  divide_slice           divideSlice           MultiAxis.cpp:13-31
+"""
+from __future__ import annotations
+
+import ctypes as _ct
+
+import numpy as _np
+
+from . import synth  # noqa: F401  (re-exported generator helpers)
+from ._lib import (  # noqa: F401
+    LIB_PATH,
+    MtrError,
+    ProjectionSize,
+    Slice,
+    VolumeSize,
+    lib,
+)
+
+PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
+FORM_FAST, FORM_PLAIN = 0, 1
+VOL_WORK, VOL_W, VOL_WBAR = 0, 1, 2
+K_FP, K_BP, K_BPT, K_CONSENSUS, K_NORMALISER, K_ELEMENTWISE, K_SPLAT = range(7)
+KERNEL_NAMES = ["fp", "bp", "bpt", "consensus", "normaliser", "elementwise", "splat"]
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib.mtr_last_error().decode(errors="replace")
+        raise MtrError(rc, f"{what}: {msg}")
+
+
+def _f32(a, shape=None) -> _np.ndarray:
+    a = _np.ascontiguousarray(a, dtype=_np.float32)
+    if shape is not None and a.size != int(_np.prod(shape)):
+        raise ValueError(f"expected {int(_np.prod(shape))} elements, got {a.size}")
+    return a
+
+
+def _f64(a) -> _np.ndarray:
+    return _np.ascontiguousarray(a, dtype=_np.float64)
+
+
+def _fp(a: _np.ndarray):
+    return a.ctypes.data_as(_ct.POINTER(_ct.c_float))
+
+
+def _dp(a: _np.ndarray):
+    return a.ctypes.data_as(_ct.POINTER(_ct.c_double))
+
+
+def volume_size(X: int, Y: int, Z: int, Xstart: int = 0, Ystart: int = 0, Zstart: int = 0) -> VolumeSize:
+    """Box [start, start+size) per axis (VolumeSize, MrcFileIOBaseFunction.h:57-70)."""
+    return VolumeSize(Xstart, Xstart + X, Ystart, Ystart + Y, Zstart, Zstart + Z, X, Y, Z)
+
+
+def projection_size(nu: int, nv: int, A: int) -> ProjectionSize:
+    return ProjectionSize(nu, nv, A)
+
+
+def divide_slice(myid: int, nparts: int, Z: int, Zstart: int = 0) -> tuple[int, int]:
+    s = Slice()
+    _check(lib.mtr_divide_slice(myid, nparts, Z, Zstart, _ct.byref(s)), "divide_slice")
+    return s.z_start, s.z_end
+
+
+class Context:
+    """One device context: resident agents, their MACE state and w-bar."""
+
+    def __init__(self, device: int = 0, precision: int = PREC_FP32):
+        self._h = _ct.c_void_p()
+        _check(lib.mtr_ctx_create(device, _ct.byref(self._h)), "mtr_ctx_create")
+        if precision != PREC_FP32:
+            _check(lib.mtr_ctx_set_precision(self._h, precision), "set_precision")
+        self.vol: VolumeSize | None = None
+        self.prjs: list[ProjectionSize] = []
+
+    # -- lifetime --------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            lib.mtr_ctx_destroy(self._h)
+            self._h = _ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- setup -------------------------------------------------------------
+    def set_volume(self, vol: VolumeSize) -> None:
+        _check(lib.mtr_ctx_set_volume(self._h, _ct.byref(vol)), "set_volume")
+        self.vol = vol
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        assert self.vol is not None
+        return (self.vol.Z, self.vol.Y, self.vol.X)
+
+    def add_agent(self, prj: ProjectionSize, xc, yc, p) -> int:
+        xc, yc = _f64(xc), _f64(yc)
+        if xc.size != 4 * prj.AngN or yc.size != 4 * prj.AngN:
+            raise ValueError("xc/yc must hold 4*AngN coefficients")
+        p = _f32(p, (prj.AngN, prj.Y, prj.X))
+        aid = _ct.c_int()
+        _check(lib.mtr_agent_add(self._h, _ct.byref(prj), _dp(xc), _dp(yc), _fp(p), _ct.byref(aid)), "agent_add")
+        self.prjs.append(prj)
+        return aid.value
+
+    def add_agent_device(self, prj: ProjectionSize, xc, yc, p_dev_ptr: int) -> int:
+        xc, yc = _f64(xc), _f64(yc)
+        aid = _ct.c_int()
+        _check(
+            lib.mtr_agent_add_device(self._h, _ct.byref(prj), _dp(xc), _dp(yc), _ct.c_void_p(p_dev_ptr), _ct.byref(aid)),
+            "agent_add_device",
+        )
+        self.prjs.append(prj)
+        return aid.value
+
+    def agent_footprint(self, prj: ProjectionSize) -> int:
+        b = _ct.c_size_t()
+        _check(lib.mtr_agent_footprint(self._h, _ct.byref(prj), _ct.byref(b)), "agent_footprint")
+        return b.value
+
+    # -- hot path ----------------------------------------------------------
+    def bpt(self, agent: int, out: bool = True):
+        o = _np.empty(self.shape, _np.float32) if out else None
+        _check(lib.mtr_bpt(self._h, agent, _fp(o) if out else None), "bpt")
+        return o
+
+    def sirt(self, agent: int, iters: int, lam: float, x0=None, out: bool = True):
+        x0p = None
+        if x0 is not None:
+            x0 = _f32(x0, self.shape)
+            x0p = _fp(x0)
+        o = _np.empty(self.shape, _np.float32) if out else None
+        _check(lib.mtr_sirt(self._h, agent, x0p, iters, _ct.c_float(lam), _fp(o) if out else None), "sirt")
+        return o
+
+    def form_z(self, agent: int) -> None:
+        _check(lib.mtr_form_z(self._h, agent), "form_z")
+
+    def mann(self, agent: int, rho: float) -> None:
+        _check(lib.mtr_mann(self._h, agent, _ct.c_float(rho)), "mann")
+
+    def mean(self) -> None:
+        _check(lib.mtr_mean(self._h), "mean")
+
+    def mace_init(self) -> None:
+        _check(lib.mtr_mace_init(self._h), "mace_init")
+
+    def mace_solve(self, M: int, S: int, lam: float, rho: float, form: int = FORM_FAST, out: bool = True):
+        o = _np.empty(self.shape, _np.float32) if out else None
+        _check(
+            lib.mtr_mace_solve(self._h, M, S, _ct.c_float(lam), _ct.c_float(rho), form, _fp(o) if out else None),
+            "mace_solve",
+        )
+        return o
+
+    def get_volume(self, which: int, agent: int = 0) -> _np.ndarray:
+        o = _np.empty(self.shape, _np.float32)
+        _check(lib.mtr_get_volume(self._h, which, agent, _fp(o)), "get_volume")
+        return o
+
+    def set_volume_data(self, which: int, agent: int, data) -> None:
+        d = _f32(data, self.shape)
+        _check(lib.mtr_set_volume(self._h, which, agent, _fp(d)), "set_volume_data")
+
+    def residual_norm2(self, agent: int) -> float:
+        v = _ct.c_double()
+        _check(lib.mtr_residual_norm2(self._h, agent, _ct.byref(v)), "residual_norm2")
+        return v.value
+
+    def consensus_diff2(self) -> float:
+        v = _ct.c_double()
+        _check(lib.mtr_consensus_diff2(self._h, _ct.byref(v)), "consensus_diff2")
+        return v.value
+
+    def sync(self) -> None:
+        _check(lib.mtr_sync(self._h), "sync")
+
+    # -- timing ------------------------------------------------------------
+    def timing(self, on: bool) -> None:
+        _check(lib.mtr_timing_enable(self._h, 1 if on else 0), "timing_enable")
+
+    def timing_reset(self) -> None:
+        _check(lib.mtr_timing_reset(self._h), "timing_reset")
+
+    def timing_get(self, kernel: int) -> tuple[int, float]:
+        n, ms = _ct.c_int64(), _ct.c_double()
+        _check(lib.mtr_timing_get(self._h, kernel, _ct.byref(n), _ct.byref(ms)), "timing_get")
+        return n.value, ms.value
+
+    # -- stateless operators -----------------------------------------------
+    def op_fp(self, prj: ProjectionSize, vol: VolumeSize, xc, yc, x):
+        xc, yc = _f64(xc), _f64(yc)
+        x = _f32(x, (vol.Z, vol.Y, vol.X))
+        s = _np.empty((prj.AngN, prj.Y, prj.X), _np.float32)
+        c = _np.empty_like(s)
+        _check(
+            lib.mtr_op_fp(self._h, _ct.byref(prj), _ct.byref(vol), _dp(xc), _dp(yc), _fp(x), _fp(s), _fp(c)), "op_fp"
+        )
+        return s, c
+
+    def op_residual(self, p, s, c):
+        p, s, c = _f32(p), _f32(s), _f32(c)
+        r = _np.empty_like(p)
+        _check(lib.mtr_op_residual(self._h, p.size, _fp(p), _fp(s), _fp(c), _fp(r)), "op_residual")
+        return r
+
+    def op_bp(self, prj: ProjectionSize, vol: VolumeSize, xc, yc, r, x, lam: float):
+        xc, yc = _f64(xc), _f64(yc)
+        r = _f32(r, (prj.AngN, prj.Y, prj.X))
+        xo = _f32(x, (vol.Z, vol.Y, vol.X)).copy()
+        _check(
+            lib.mtr_op_bp(self._h, _ct.byref(prj), _ct.byref(vol), _dp(xc), _dp(yc), _fp(r), _fp(xo), _ct.c_float(lam)),
+            "op_bp",
+        )
+        return xo
+
+    def op_bpt(self, prj: ProjectionSize, vol: VolumeSize, xc, yc, p):
+        xc, yc = _f64(xc), _f64(yc)
+        p = _f32(p, (prj.AngN, prj.Y, prj.X))
+        xo = _np.empty((vol.Z, vol.Y, vol.X), _np.float32)
+        _check(lib.mtr_op_bpt(self._h, _ct.byref(prj), _ct.byref(vol), _dp(xc), _dp(yc), _fp(p), _fp(xo)), "op_bpt")
+        return xo
+
+
+def gpu_available() -> bool:
+    """True when the library can open a context on device 0."""
+    h = _ct.c_void_p()
+    if lib.mtr_ctx_create(0, _ct.byref(h)) != 0:
+        return False
+    lib.mtr_ctx_destroy(h)
+    return True
+
+
+__all__ = [
+    "Context",
+    "MtrError",
+    "VolumeSize",
+    "ProjectionSize",
+    "Slice",
+    "volume_size",
+    "projection_size",
+    "divide_slice",
+    "gpu_available",
+    "synth",
+    "LIB_PATH",
+]

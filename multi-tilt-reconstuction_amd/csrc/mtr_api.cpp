@@ -1,0 +1,1024 @@
+// Host side of the C ABI: device context, resident agents, the MACE solve
+// loop and the stateless operators.  Compiled with hipcc (host code only).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mtr.h"
+#include "mtr_kernels.h"
+
+using namespace mtr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? MTR_ERR_OOM : MTR_ERR_HIP, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+#define MTR_TRY(expr)             \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != MTR_OK) return rc_; \
+    } while (0)
+
+size_t elem_size(int prec) { return prec == MTR_PREC_FP32 ? 4 : 2; }
+
+bool vol_valid(const mtr_volume_size *v)
+{
+    return v && v->X > 0 && v->Y > 0 && v->Z > 0 && v->Xend - v->Xstart == v->X &&
+           v->Yend - v->Ystart == v->Y && v->Zend - v->Zstart == v->Z;
+}
+
+bool prj_valid(const mtr_projection_size *p)
+{
+    return p && p->X > 0 && p->Y > 0 && p->AngN > 0 && p->AngN <= 100000;
+}
+
+// Builds the ray-driven forward-projection plan of one view.
+int make_plan(const double *xc, const double *yc, FpPlan &pl)
+{
+    std::memset(&pl, 0, sizeof pl);
+    const double a[3] = {xc[1], xc[2], xc[3]};
+    const double b[3] = {yc[1], yc[2], yc[3]};
+    if (!std::isfinite(xc[0]) || !std::isfinite(yc[0])) return MTR_ERR_NONFINITE;
+    for (int m = 0; m < 3; m++)
+        if (!std::isfinite(a[m]) || !std::isfinite(b[m])) return MTR_ERR_NONFINITE;
+    // Row-separable: v = yc0 + bW*W - 1 with bW = +-1, the other b zero and
+    // yc0 integral -> every integer voxel lands on an integer row.
+    int nzb = 0, W = -1;
+    for (int m = 0; m < 3; m++)
+        if (b[m] != 0.0) {
+            nzb++;
+            W = m;
+        }
+    if (nzb == 1 && std::fabs(b[W]) == 1.0 && std::floor(yc[0]) == yc[0] &&
+        std::fabs(yc[0]) < 1e9) {
+        const int A1 = W == 0 ? 1 : 0, A2 = W == 2 ? 1 : 2;
+        // Line (drive) axis: the in-plane axis with the smaller |a|.
+        int L = A1, M = A2;
+        if (std::fabs(a[A1]) > std::fabs(a[A2])) {
+            L = A2;
+            M = A1;
+        }
+        if (a[M] != 0.0) {
+            pl.kind = 1;
+            pl.W = W;
+            pl.L = L;
+            pl.M = M;
+            pl.wsign = b[W] > 0 ? 1 : -1;
+            pl.woff = (1.0 - yc[0]) * pl.wsign;
+            pl.m0 = (1.0 - xc[0]) / a[M];
+            pl.mi = 1.0 / a[M];
+            pl.mw = -a[W] / a[M];
+            pl.ml = -a[L] / a[M];
+            pl.Em = 1.0 / std::fabs(a[M]);
+            return MTR_OK;
+        }
+    }
+    // General plan: driving axis = largest component of grad u x grad v.
+    const double d[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    int D = 0;
+    for (int m = 1; m < 3; m++)
+        if (std::fabs(d[m]) > std::fabs(d[D])) D = m;
+    if (!(std::fabs(d[D]) > 0.0)) return MTR_ERR_SHAPE;
+    const int P = D == 0 ? 1 : 0, Q = D == 2 ? 1 : 2;
+    const double det = a[P] * b[Q] - a[Q] * b[P];
+    if (!(std::fabs(det) > 0.0)) return MTR_ERR_SHAPE;
+    const double m00 = b[Q] / det, m01 = -a[Q] / det, m10 = -b[P] / det, m11 = a[P] / det;
+    pl.kind = 0;
+    pl.D = D;
+    pl.P = P;
+    pl.Q = Q;
+    pl.p0 = m00 * (1.0 - xc[0]) + m01 * (1.0 - yc[0]);
+    pl.pi = m00;
+    pl.pk = m01;
+    pl.pd = -(m00 * a[D] + m01 * b[D]);
+    pl.q0 = m10 * (1.0 - xc[0]) + m11 * (1.0 - yc[0]);
+    pl.qi = m10;
+    pl.qk = m11;
+    pl.qd = -(m10 * a[D] + m11 * b[D]);
+    pl.Ep = std::fabs(m00) + std::fabs(m01);
+    pl.Eq = std::fabs(m10) + std::fabs(m11);
+    return MTR_OK;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+};
+
+struct Agent {
+    mtr_projection_size prj{};
+    std::vector<double> xc, yc;
+    bool lanes_k = false;  // forward projection with lanes along detector rows
+    ViewCoef *d_coef = nullptr;
+    FpPlan *d_plan = nullptr;
+    void *d_p = nullptr;
+    bool p_owned = true;
+    float *d_c = nullptr;
+    void *d_r = nullptr;
+    void *d_x = nullptr;
+    void *d_w = nullptr;
+    double *d_partial = nullptr;
+    double *d_scalar = nullptr;  // reduced ||r||^2
+    int nfp = 0;
+    bool c_ready = false;
+    int64_t P() const { return (int64_t)prj.X * prj.Y * prj.AngN; }
+};
+
+}  // namespace
+
+struct mtr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = MTR_PREC_FP32;
+    mtr_volume_size vol{};
+    bool vol_set = false;
+    int64_t N = 0;
+    void *d_wbar = nullptr;
+    std::vector<Agent> agents;
+    AgentDev *d_ags = nullptr;  // table in agent order
+    double *d_cons_partial = nullptr;
+    double *d_cons_scalar = nullptr;
+    int cons_blocks = 0;
+    // plain form scratch
+    float *d_s = nullptr, *d_cc = nullptr;
+    int64_t sc_cap = 0;
+    // tuning
+    int bp_tz = 8;
+    int lds_cap = 6144;
+    // timing
+    bool timing = false;
+    struct Pending {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    int64_t launches[MTR_K_COUNT] = {0};
+    double ms[MTR_K_COUNT] = {0};
+};
+
+namespace {
+
+int set_device(mtr_ctx *ctx)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    return MTR_OK;
+}
+
+hipEvent_t take_event(mtr_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets one launch with events when timing is on.
+struct Timed {
+    mtr_ctx *ctx;
+    int kernel;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(mtr_ctx *c, int k) : ctx(c), kernel(k)
+    {
+        if (ctx->timing) {
+            a = take_event(ctx);
+            b = take_event(ctx);
+            if (a && b) (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Timed()
+    {
+        if (a && b) {
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->pending.push_back({kernel, a, b});
+        }
+    }
+};
+
+int collect_timing(mtr_ctx *ctx)
+{
+    if (ctx->pending.empty()) return MTR_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &pd : ctx->pending) {
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, pd.a, pd.b));
+        ctx->launches[pd.kernel]++;
+        ctx->ms[pd.kernel] += t;
+        ctx->event_pool.push_back(pd.a);
+        ctx->event_pool.push_back(pd.b);
+    }
+    ctx->pending.clear();
+    return MTR_OK;
+}
+
+AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
+{
+    AgentDev d;
+    d.nu = ag.prj.X;
+    d.nv = ag.prj.Y;
+    d.A = ag.prj.AngN;
+    d.Xs = ctx->vol.Xstart;
+    d.Xe = ctx->vol.Xend;
+    d.Ys = ctx->vol.Ystart;
+    d.Ye = ctx->vol.Yend;
+    d.Zs = ctx->vol.Zstart;
+    d.Ze = ctx->vol.Zend;
+    d.X = ctx->vol.X;
+    d.Y = ctx->vol.Y;
+    d.Z = ctx->vol.Z;
+    d.coef = ag.d_coef;
+    d.plan = ag.d_plan;
+    d.p = ag.d_p;
+    d.c = ag.d_c;
+    d.r = ag.d_r;
+    d.x = ag.d_x;
+    d.partial = ag.d_partial;
+    return d;
+}
+
+int upload_table(mtr_ctx *ctx)
+{
+    std::vector<AgentDev> t;
+    for (auto &ag : ctx->agents) t.push_back(make_dev(ctx, ag));
+    if (t.empty()) return MTR_OK;
+    HIP_TRY(hipMemcpyAsync(ctx->d_ags, t.data(), sizeof(AgentDev) * t.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int check_agent(mtr_ctx *ctx, int agent)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (agent < 0 || agent >= (int)ctx->agents.size())
+        return fail(MTR_ERR_RANGE, "agent %d out of range (have %zu)", agent, ctx->agents.size());
+    return MTR_OK;
+}
+
+// Geometry-only forward-projection normaliser of one agent.
+int ensure_normaliser(mtr_ctx *ctx, int a)
+{
+    Agent &ag = ctx->agents[a];
+    if (ag.c_ready) return MTR_OK;
+    {
+        Timed t(ctx, MTR_K_NORMALISER);
+        HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + a, 1, ag.nfp, 2, ag.lanes_k, nullptr, ag.d_c,
+                          ctx->stream));
+    }
+    ag.c_ready = true;
+    return MTR_OK;
+}
+
+// One fused FP+residual launch over a set of agents sharing a lane mode.
+int fp_group(mtr_ctx *ctx, const std::vector<int> &ids)
+{
+    // Agents are contiguous in the device table only by index; launch each
+    // maximal run of consecutive ids with the same lane mode together.
+    size_t s = 0;
+    while (s < ids.size()) {
+        size_t e = s + 1;
+        while (e < ids.size() && ids[e] == ids[e - 1] + 1 &&
+               ctx->agents[ids[e]].lanes_k == ctx->agents[ids[s]].lanes_k)
+            e++;
+        int mb = 0;
+        for (size_t q = s; q < e; q++) mb = std::max(mb, ctx->agents[ids[q]].nfp);
+        Timed t(ctx, MTR_K_FP);
+        HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, 0,
+                          ctx->agents[ids[s]].lanes_k, nullptr, nullptr, ctx->stream));
+        s = e;
+    }
+    return MTR_OK;
+}
+
+int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
+{
+    const int mb = bp_blocks(make_dev(ctx, ctx->agents[first]), ctx->bp_tz);
+    Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
+    HIP_TRY(launch_bp(ctx->precision, ctx->d_ags + first, count, mb, ctx->bp_tz, bpt, lambda, nullptr,
+                      ctx->lds_cap, ctx->stream));
+    return MTR_OK;
+}
+
+int ensure_plain_scratch(mtr_ctx *ctx, int64_t P)
+{
+    if (ctx->sc_cap >= P) return MTR_OK;
+    if (ctx->d_s) (void)hipFree(ctx->d_s);
+    if (ctx->d_cc) (void)hipFree(ctx->d_cc);
+    ctx->d_s = ctx->d_cc = nullptr;
+    ctx->sc_cap = 0;
+    HIP_TRY(hipMalloc(&ctx->d_s, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&ctx->d_cc, sizeof(float) * P));
+    ctx->sc_cap = P;
+    return MTR_OK;
+}
+
+// Plain form: one agent, splat FP with atomics into zeroed s/c (c
+// recomputed every iteration), separate residual kernel, direct-gather BP,
+// host synchronisation after each iteration.
+int sirt_plain(mtr_ctx *ctx, int a, int iters, float lambda)
+{
+    Agent &ag = ctx->agents[a];
+    if (ctx->precision != MTR_PREC_FP32)
+        return fail(MTR_ERR_STATE, "plain form runs FP32 storage only");
+    MTR_TRY(ensure_plain_scratch(ctx, ag.P()));
+    const AgentDev d = make_dev(ctx, ag);
+    for (int it = 0; it < iters; it++) {
+        HIP_TRY(hipMemsetAsync(ctx->d_s, 0, sizeof(float) * ag.P(), ctx->stream));
+        HIP_TRY(hipMemsetAsync(ctx->d_cc, 0, sizeof(float) * ag.P(), ctx->stream));
+        {
+            Timed t(ctx, MTR_K_SPLAT);
+            HIP_TRY(launch_plain_splat(d, ctx->d_s, ctx->d_cc, ctx->stream));
+        }
+        {
+            Timed t(ctx, MTR_K_ELEMENTWISE);
+            HIP_TRY(launch_residual(ctx->precision, ag.P(), ag.d_p, ctx->d_s, ctx->d_cc, ag.d_r, ctx->stream));
+        }
+        {
+            Timed t(ctx, MTR_K_BP);
+            HIP_TRY(launch_plain_bp(d, (const float *)ag.d_r, lambda, 0, ctx->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    return MTR_OK;
+}
+
+int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda)
+{
+    for (int a : ids) MTR_TRY(ensure_normaliser(ctx, a));
+    // agents of a context are contiguous from 0; BP launches the whole run.
+    for (int it = 0; it < iters; it++) {
+        MTR_TRY(fp_group(ctx, ids));
+        size_t s = 0;
+        while (s < ids.size()) {
+            size_t e = s + 1;
+            while (e < ids.size() && ids[e] == ids[e - 1] + 1) e++;
+            MTR_TRY(bp_group(ctx, ids[s], (int)(e - s), false, lambda));
+            s = e;
+        }
+    }
+    return MTR_OK;
+}
+
+int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc, const double *yc,
+                     const float *p_host, const float *p_dev, int *agent_id)
+{
+    if (!ctx || !prj || !xc || !yc || (!p_host && !p_dev)) return fail(MTR_ERR_ARG, "null argument");
+    if (!ctx->vol_set) return fail(MTR_ERR_STATE, "mtr_ctx_set_volume must precede mtr_agent_add");
+    if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size %d x %d x %d", prj->X, prj->Y, prj->AngN);
+    if ((int)ctx->agents.size() >= kMaxAgents)
+        return fail(MTR_ERR_RANGE, "at most %d agents per context", kMaxAgents);
+    MTR_TRY(set_device(ctx));
+    const int A = prj->AngN;
+    std::vector<ViewCoef> coef(A);
+    std::vector<FpPlan> plan(A);
+    for (int v = 0; v < A; v++) {
+        for (int m = 0; m < 4; m++) {
+            coef[v].c[m] = xc[4 * v + m];
+            coef[v].c[4 + m] = yc[4 * v + m];
+        }
+        int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
+        if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
+    }
+    ctx->agents.emplace_back();
+    Agent &ag = ctx->agents.back();
+    auto undo = [&](int rc) {
+        Agent &g = ctx->agents.back();
+        for (void *q : {(void *)g.d_coef, (void *)g.d_plan, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
+                        g.d_r, g.d_x, g.d_w, (void *)g.d_partial, (void *)g.d_scalar})
+            if (q) (void)hipFree(q);
+        ctx->agents.pop_back();
+        return rc;
+    };
+    ag.prj = *prj;
+    ag.xc.assign(xc, xc + 4 * A);
+    ag.yc.assign(yc, yc + 4 * A);
+    // Lanes along detector rows when every row is one X plane: the 64
+    // lanes of a wave then walk the same (Y, Z) path in adjacent planes.
+    bool all_wx = true;
+    for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
+    ag.lanes_k = all_wx;
+    const size_t es = elem_size(ctx->precision);
+    const int64_t P = ag.P();
+#define ALLOC(ptr, bytes)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                              \
+        if (e_ != hipSuccess)                                                             \
+            return undo(fail(MTR_ERR_OOM, "hipMalloc %zu bytes: %s", (size_t)(bytes),     \
+                             hipGetErrorString(e_)));                                     \
+    } while (0)
+    ALLOC(ag.d_coef, sizeof(ViewCoef) * A);
+    ALLOC(ag.d_plan, sizeof(FpPlan) * A);
+    ALLOC(ag.d_c, sizeof(float) * P);
+    ALLOC(ag.d_r, es * P);
+    ALLOC(ag.d_x, es * ctx->N);
+    ALLOC(ag.d_w, es * ctx->N);
+    ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
+    ALLOC(ag.d_partial, sizeof(double) * std::max(ag.nfp, 1));
+    ALLOC(ag.d_scalar, sizeof(double));
+    if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(ag.d_plan, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess)
+        return undo(fail(MTR_ERR_HIP, "geometry upload failed"));
+    if (ctx->precision == MTR_PREC_FP32 && p_dev) {
+        ag.d_p = const_cast<float *>(p_dev);
+        ag.p_owned = false;
+    } else {
+        ALLOC(ag.d_p, es * P);
+        ag.p_owned = true;
+        if (p_host && ctx->precision == MTR_PREC_FP32) {
+            if (hipMemcpyAsync(ag.d_p, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+                return undo(fail(MTR_ERR_HIP, "projection upload failed"));
+        } else {
+            const float *src = p_dev;
+            float *tmp = nullptr;
+            if (p_host) {
+                if (hipMalloc(&tmp, sizeof(float) * P) != hipSuccess)
+                    return undo(fail(MTR_ERR_OOM, "staging buffer"));
+                if (hipMemcpyAsync(tmp, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+                    (void)hipFree(tmp);
+                    return undo(fail(MTR_ERR_HIP, "projection upload failed"));
+                }
+                src = tmp;
+            }
+            hipError_t e = launch_to_storage(ctx->precision, P, src, ag.d_p, ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
+            if (tmp) (void)hipFree(tmp);
+            if (e != hipSuccess) return undo(fail(MTR_ERR_HIP, "projection conversion failed"));
+        }
+    }
+#undef ALLOC
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return undo(fail(MTR_ERR_HIP, "upload sync failed"));
+    if (launch_fill(ctx->precision, ctx->N, ag.d_x, 0.f, ctx->stream) != hipSuccess ||
+        launch_fill(ctx->precision, ctx->N, ag.d_w, 0.f, ctx->stream) != hipSuccess)
+        return undo(fail(MTR_ERR_HIP, "volume init failed"));
+    int rc = upload_table(ctx);
+    if (rc != MTR_OK) return undo(rc);
+    if (agent_id) *agent_id = (int)ctx->agents.size() - 1;
+    return MTR_OK;
+}
+
+// Temporary single-agent device state for the stateless operators (FP32).
+struct TmpAgent {
+    DevBuf coef, plan, x, s, c, r, tab;
+    AgentDev dev{};
+    bool lanes_k = false;
+    int nfp = 0;
+};
+
+int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol, const double *xc,
+              const double *yc, TmpAgent &t)
+{
+    if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
+    if (!vol_valid(vol)) return fail(MTR_ERR_SHAPE, "invalid volume size");
+    const int A = prj->AngN;
+    std::vector<ViewCoef> coef(A);
+    std::vector<FpPlan> plan(A);
+    bool all_wx = true;
+    for (int v = 0; v < A; v++) {
+        for (int m = 0; m < 4; m++) {
+            coef[v].c[m] = xc[4 * v + m];
+            coef[v].c[4 + m] = yc[4 * v + m];
+        }
+        int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
+        if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
+        all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
+    }
+    t.lanes_k = all_wx;
+    const int64_t P = (int64_t)prj->X * prj->Y * A;
+    const int64_t N = (int64_t)vol->X * vol->Y * vol->Z;
+    HIP_TRY(hipMalloc(&t.coef.p, sizeof(ViewCoef) * A));
+    HIP_TRY(hipMalloc(&t.plan.p, sizeof(FpPlan) * A));
+    HIP_TRY(hipMalloc(&t.x.p, sizeof(float) * N));
+    HIP_TRY(hipMalloc(&t.s.p, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&t.c.p, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&t.r.p, sizeof(float) * P));
+    HIP_TRY(hipMalloc(&t.tab.p, sizeof(AgentDev)));
+    HIP_TRY(hipMemcpyAsync(t.coef.p, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(t.plan.p, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream));
+    AgentDev &d = t.dev;
+    d.nu = prj->X;
+    d.nv = prj->Y;
+    d.A = A;
+    d.Xs = vol->Xstart;
+    d.Xe = vol->Xend;
+    d.Ys = vol->Ystart;
+    d.Ye = vol->Yend;
+    d.Zs = vol->Zstart;
+    d.Ze = vol->Zend;
+    d.X = vol->X;
+    d.Y = vol->Y;
+    d.Z = vol->Z;
+    d.coef = (const ViewCoef *)t.coef.p;
+    d.plan = (const FpPlan *)t.plan.p;
+    d.p = nullptr;
+    d.c = (const float *)t.c.p;
+    d.r = t.r.p;
+    d.x = t.x.p;
+    d.partial = nullptr;
+    HIP_TRY(hipMemcpyAsync(t.tab.p, &d, sizeof(AgentDev), hipMemcpyHostToDevice, ctx->stream));
+    t.nfp = fp_blocks(d, t.lanes_k);
+    return MTR_OK;
+}
+
+void *vol_ptr(mtr_ctx *ctx, int which, int agent)
+{
+    if (which == MTR_VOL_WBAR) return ctx->d_wbar;
+    Agent &ag = ctx->agents[agent];
+    return which == MTR_VOL_WORK ? ag.d_x : ag.d_w;
+}
+
+int copy_volume_out(mtr_ctx *ctx, const void *dev, float *host)
+{
+    if (ctx->precision == MTR_PREC_FP32) {
+        HIP_TRY(hipMemcpyAsync(host, dev, sizeof(float) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return MTR_OK;
+    }
+    DevBuf tmp;
+    HIP_TRY(hipMalloc(&tmp.p, sizeof(float) * ctx->N));
+    HIP_TRY(launch_from_storage(ctx->precision, ctx->N, dev, (float *)tmp.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(host, tmp.p, sizeof(float) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int copy_volume_in(mtr_ctx *ctx, void *dev, const float *host)
+{
+    if (ctx->precision == MTR_PREC_FP32) {
+        HIP_TRY(hipMemcpyAsync(dev, host, sizeof(float) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return MTR_OK;
+    }
+    DevBuf tmp;
+    HIP_TRY(hipMalloc(&tmp.p, sizeof(float) * ctx->N));
+    HIP_TRY(hipMemcpyAsync(tmp.p, host, sizeof(float) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_to_storage(ctx->precision, ctx->N, (const float *)tmp.p, dev, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+}  // namespace
+
+// ======================================================================
+extern "C" {
+
+int mtr_abi_version(void) { return MTR_ABI_VERSION; }
+
+const char *mtr_last_error(void) { return g_err.c_str(); }
+
+int mtr_ctx_create(int device, mtr_ctx **out)
+{
+    if (!out) return fail(MTR_ERR_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MTR_ERR_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(MTR_ERR_RANGE, "device %d out of range (%d visible)", device, n);
+    mtr_ctx *ctx = new mtr_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return fail(MTR_ERR_HIP, "stream creation failed");
+    }
+    if (hipMalloc(&ctx->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&ctx->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
+        hipMalloc(&ctx->d_cons_scalar, sizeof(double)) != hipSuccess) {
+        mtr_ctx_destroy(ctx);
+        return fail(MTR_ERR_OOM, "context tables");
+    }
+    *out = ctx;
+    return MTR_OK;
+}
+
+int mtr_ctx_destroy(mtr_ctx *ctx)
+{
+    if (!ctx) return MTR_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &ag : ctx->agents) {
+        for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
+                        ag.d_x, ag.d_w, (void *)ag.d_partial, (void *)ag.d_scalar})
+            if (q) (void)hipFree(q);
+    }
+    for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
+                    (void *)ctx->d_s, (void *)ctx->d_cc})
+        if (q) (void)hipFree(q);
+    for (auto &pd : ctx->pending) {
+        (void)hipEventDestroy(pd.a);
+        (void)hipEventDestroy(pd.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MTR_OK;
+}
+
+int mtr_ctx_set_precision(mtr_ctx *ctx, int precision)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (precision != MTR_PREC_FP32 && precision != MTR_PREC_BF16 && precision != MTR_PREC_FP16)
+        return fail(MTR_ERR_ARG, "unknown precision %d", precision);
+    if (ctx->vol_set) return fail(MTR_ERR_STATE, "precision must be set before the volume");
+    ctx->precision = precision;
+    return MTR_OK;
+}
+
+int mtr_ctx_set_volume(mtr_ctx *ctx, const mtr_volume_size *vol)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (!vol_valid(vol)) return fail(MTR_ERR_SHAPE, "invalid volume box");
+    if (ctx->vol_set) return fail(MTR_ERR_STATE, "volume already set");
+    MTR_TRY(set_device(ctx));
+    ctx->vol = *vol;
+    ctx->N = (int64_t)vol->X * vol->Y * vol->Z;
+    HIP_TRY(hipMalloc(&ctx->d_wbar, elem_size(ctx->precision) * ctx->N));
+    HIP_TRY(launch_fill(ctx->precision, ctx->N, ctx->d_wbar, 0.f, ctx->stream));
+    ctx->vol_set = true;
+    return MTR_OK;
+}
+
+int mtr_agent_add(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc, const double *yc,
+                  const float *p_host, int *agent_id)
+{
+    return agent_add_common(ctx, prj, xc, yc, p_host, nullptr, agent_id);
+}
+
+int mtr_agent_add_device(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc, const double *yc,
+                         const float *p_dev, int *agent_id)
+{
+    return agent_add_common(ctx, prj, xc, yc, nullptr, p_dev, agent_id);
+}
+
+int mtr_agent_count(const mtr_ctx *ctx, int *count)
+{
+    if (!ctx || !count) return fail(MTR_ERR_ARG, "null argument");
+    *count = (int)ctx->agents.size();
+    return MTR_OK;
+}
+
+int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size_t *bytes)
+{
+    if (!ctx || !bytes) return fail(MTR_ERR_ARG, "null argument");
+    if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
+    if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
+    const size_t es = elem_size(ctx->precision);
+    const size_t P = (size_t)prj->X * prj->Y * prj->AngN;
+    const size_t nfp = (size_t)((prj->X + 63) / 64 + 16) * ((prj->Y + 3) / 4 + 16) * prj->AngN;
+    *bytes = sizeof(ViewCoef) * prj->AngN + sizeof(FpPlan) * prj->AngN + es * P /*p*/ + 4 * P /*c*/ +
+             es * P /*r*/ + 2 * es * (size_t)ctx->N /*x, w*/ + 8 * nfp;
+    return MTR_OK;
+}
+
+int mtr_bpt(mtr_ctx *ctx, int agent, float *out_host)
+{
+    MTR_TRY(check_agent(ctx, agent));
+    MTR_TRY(set_device(ctx));
+    MTR_TRY(bp_group(ctx, agent, 1, true, 1.0f));
+    if (out_host) MTR_TRY(copy_volume_out(ctx, ctx->agents[agent].d_x, out_host));
+    return collect_timing(ctx);
+}
+
+int mtr_sirt(mtr_ctx *ctx, int agent, const float *x0_host, int iters, float lambda, float *out_host)
+{
+    MTR_TRY(check_agent(ctx, agent));
+    if (iters < 0) return fail(MTR_ERR_ARG, "iters < 0");
+    MTR_TRY(set_device(ctx));
+    if (x0_host) MTR_TRY(copy_volume_in(ctx, ctx->agents[agent].d_x, x0_host));
+    MTR_TRY(sirt_fast(ctx, {agent}, iters, lambda));
+    if (out_host) MTR_TRY(copy_volume_out(ctx, ctx->agents[agent].d_x, out_host));
+    return collect_timing(ctx);
+}
+
+int mtr_form_z(mtr_ctx *ctx, int agent)
+{
+    MTR_TRY(check_agent(ctx, agent));
+    MTR_TRY(set_device(ctx));
+    Agent &ag = ctx->agents[agent];
+    {
+        Timed t(ctx, MTR_K_ELEMENTWISE);
+        HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, ag.d_x, ctx->stream));
+    }
+    return collect_timing(ctx);
+}
+
+int mtr_mann(mtr_ctx *ctx, int agent, float rho)
+{
+    MTR_TRY(check_agent(ctx, agent));
+    MTR_TRY(set_device(ctx));
+    Agent &ag = ctx->agents[agent];
+    {
+        Timed t(ctx, MTR_K_ELEMENTWISE);
+        HIP_TRY(launch_mann_z(ctx->precision, ctx->N, ag.d_x, ag.d_w, ctx->d_wbar, rho, ctx->stream));
+    }
+    return collect_timing(ctx);
+}
+
+int mtr_mean(mtr_ctx *ctx)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    MTR_TRY(set_device(ctx));
+    std::vector<const void *> w;
+    for (auto &ag : ctx->agents) w.push_back(ag.d_w);
+    {
+        Timed t(ctx, MTR_K_ELEMENTWISE);
+        HIP_TRY(launch_mean(ctx->precision, (int)w.size(), ctx->N, w.data(), ctx->d_wbar, ctx->stream));
+    }
+    return collect_timing(ctx);
+}
+
+int mtr_mace_init(mtr_ctx *ctx)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    MTR_TRY(set_device(ctx));
+    const int K = (int)ctx->agents.size();
+    // BPT of every agent into its working volume, one batched launch.
+    MTR_TRY(bp_group(ctx, 0, K, true, 1.0f));
+    std::vector<void *> w, x;
+    for (auto &ag : ctx->agents) {
+        w.push_back(ag.d_w);
+        x.push_back(ag.d_x);
+    }
+    {
+        Timed t(ctx, MTR_K_ELEMENTWISE);
+        HIP_TRY(launch_init_sum(ctx->precision, K, ctx->N, w.data(), x.data(), ctx->d_wbar, ctx->stream));
+    }
+    return MTR_OK;
+}
+
+int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form, float *wbar_host)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    if (M < 0 || S < 0) return fail(MTR_ERR_ARG, "negative iteration count");
+    if (form != MTR_FORM_FAST && form != MTR_FORM_PLAIN) return fail(MTR_ERR_ARG, "unknown form %d", form);
+    MTR_TRY(set_device(ctx));
+    const int K = (int)ctx->agents.size();
+    if (form == MTR_FORM_PLAIN) {
+        if (ctx->precision != MTR_PREC_FP32) return fail(MTR_ERR_STATE, "plain form runs FP32 storage only");
+        // BPT per agent, sum, then per step: z, SIRT, Mann per agent, mean.
+        for (int a = 0; a < K; a++) {
+            Timed t(ctx, MTR_K_BPT);
+            HIP_TRY(launch_plain_bp(make_dev(ctx, ctx->agents[a]), (const float *)ctx->agents[a].d_p, 1.0f, 1,
+                                    ctx->stream));
+        }
+        std::vector<void *> w, x;
+        for (auto &ag : ctx->agents) {
+            w.push_back(ag.d_w);
+            x.push_back(ag.d_x);
+        }
+        {
+            Timed t(ctx, MTR_K_ELEMENTWISE);
+            HIP_TRY(launch_init_sum(ctx->precision, K, ctx->N, w.data(), x.data(), ctx->d_wbar, ctx->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        for (int m = 0; m < M; m++) {
+            for (int a = 0; a < K; a++) {
+                Agent &ag = ctx->agents[a];
+                {
+                    Timed t(ctx, MTR_K_ELEMENTWISE);
+                    HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, ag.d_x, ctx->stream));
+                }
+                MTR_TRY(sirt_plain(ctx, a, S, lambda));
+                {
+                    Timed t(ctx, MTR_K_ELEMENTWISE);
+                    HIP_TRY(launch_mann_z(ctx->precision, ctx->N, ag.d_x, ag.d_w, ctx->d_wbar, rho, ctx->stream));
+                }
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+            }
+            std::vector<const void *> wc(w.begin(), w.end());
+            {
+                Timed t(ctx, MTR_K_ELEMENTWISE);
+                HIP_TRY(launch_mean(ctx->precision, K, ctx->N, wc.data(), ctx->d_wbar, ctx->stream));
+            }
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+    } else {
+        MTR_TRY(mtr_mace_init(ctx));
+        std::vector<int> ids(K);
+        for (int a = 0; a < K; a++) ids[a] = a;
+        std::vector<void *> w, x;
+        for (auto &ag : ctx->agents) {
+            w.push_back(ag.d_w);
+            x.push_back(ag.d_x);
+        }
+        const int cb = consensus_blocks(ctx->N);
+        for (int m = 0; m < M; m++) {
+            MTR_TRY(sirt_fast(ctx, ids, S, lambda));
+            Timed t(ctx, MTR_K_CONSENSUS);
+            HIP_TRY(launch_consensus(ctx->precision, K, ctx->N, rho, 1, w.data(), x.data(), ctx->d_wbar,
+                                     ctx->d_cons_partial, cb, ctx->stream));
+        }
+        ctx->cons_blocks = M > 0 ? cb : 0;
+    }
+    if (wbar_host) MTR_TRY(copy_volume_out(ctx, ctx->d_wbar, wbar_host));
+    return collect_timing(ctx);
+}
+
+int mtr_get_volume(mtr_ctx *ctx, int which, int agent, float *host)
+{
+    if (!ctx || !host) return fail(MTR_ERR_ARG, "null argument");
+    if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
+    if (which != MTR_VOL_WBAR) MTR_TRY(check_agent(ctx, agent));
+    if (which < 0 || which > 2) return fail(MTR_ERR_ARG, "unknown volume %d", which);
+    MTR_TRY(set_device(ctx));
+    return copy_volume_out(ctx, vol_ptr(ctx, which, agent), host);
+}
+
+int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host)
+{
+    if (!ctx || !host) return fail(MTR_ERR_ARG, "null argument");
+    if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
+    if (which != MTR_VOL_WBAR) MTR_TRY(check_agent(ctx, agent));
+    if (which < 0 || which > 2) return fail(MTR_ERR_ARG, "unknown volume %d", which);
+    MTR_TRY(set_device(ctx));
+    return copy_volume_in(ctx, vol_ptr(ctx, which, agent), host);
+}
+
+int mtr_wbar_device(mtr_ctx *ctx, void **dev_ptr)
+{
+    if (!ctx || !dev_ptr) return fail(MTR_ERR_ARG, "null argument");
+    if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
+    *dev_ptr = ctx->d_wbar;
+    return MTR_OK;
+}
+
+int mtr_residual_norm2(mtr_ctx *ctx, int agent, double *out)
+{
+    MTR_TRY(check_agent(ctx, agent));
+    if (!out) return fail(MTR_ERR_ARG, "null out");
+    MTR_TRY(set_device(ctx));
+    Agent &ag = ctx->agents[agent];
+    HIP_TRY(launch_reduce(ag.d_partial, ag.nfp, ag.d_scalar, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, ag.d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int mtr_consensus_diff2(mtr_ctx *ctx, double *out)
+{
+    if (!ctx || !out) return fail(MTR_ERR_ARG, "null argument");
+    if (ctx->cons_blocks <= 0) return fail(MTR_ERR_STATE, "no consensus step has run");
+    MTR_TRY(set_device(ctx));
+    HIP_TRY(launch_reduce(ctx->d_cons_partial, ctx->cons_blocks, ctx->d_cons_scalar, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, ctx->d_cons_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int mtr_op_fp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol, const double *xc,
+              const double *yc, const float *x, float *s, float *c)
+{
+    if (!ctx || !prj || !vol || !xc || !yc || !x || !s || !c) return fail(MTR_ERR_ARG, "null argument");
+    MTR_TRY(set_device(ctx));
+    TmpAgent t;
+    MTR_TRY(tmp_agent(ctx, prj, vol, xc, yc, t));
+    const int64_t P = (int64_t)prj->X * prj->Y * prj->AngN, N = (int64_t)vol->X * vol->Y * vol->Z;
+    HIP_TRY(hipMemcpyAsync(t.x.p, x, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
+    {
+        Timed tm(ctx, MTR_K_FP);
+        HIP_TRY(launch_fp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, t.nfp, 1, t.lanes_k, (float *)t.s.p,
+                          (float *)t.c.p, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(s, t.s.p, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(c, t.c.p, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return collect_timing(ctx);
+}
+
+int mtr_op_residual(mtr_ctx *ctx, int64_t n, const float *p, const float *s, const float *c, float *r)
+{
+    if (!ctx || !p || !s || !c || !r || n < 0) return fail(MTR_ERR_ARG, "bad argument");
+    if (n == 0) return MTR_OK;
+    MTR_TRY(set_device(ctx));
+    DevBuf dp, ds, dc, dr;
+    HIP_TRY(hipMalloc(&dp.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&ds.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&dc.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&dr.p, sizeof(float) * n));
+    HIP_TRY(hipMemcpyAsync(dp.p, p, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ds.p, s, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dc.p, c, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_residual(MTR_PREC_FP32, n, dp.p, (const float *)ds.p, (const float *)dc.p, dr.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(r, dr.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+static int op_bp_common(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
+                        const double *xc, const double *yc, const float *r, float *x, float lambda, bool bpt)
+{
+    if (!ctx || !prj || !vol || !xc || !yc || !r || !x) return fail(MTR_ERR_ARG, "null argument");
+    MTR_TRY(set_device(ctx));
+    TmpAgent t;
+    MTR_TRY(tmp_agent(ctx, prj, vol, xc, yc, t));
+    const int64_t P = (int64_t)prj->X * prj->Y * prj->AngN, N = (int64_t)vol->X * vol->Y * vol->Z;
+    HIP_TRY(hipMemcpyAsync(t.r.p, r, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream));
+    if (!bpt) HIP_TRY(hipMemcpyAsync(t.x.p, x, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
+    {
+        Timed tm(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
+        HIP_TRY(launch_bp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, bp_blocks(t.dev, ctx->bp_tz), ctx->bp_tz,
+                          bpt, lambda, nullptr, ctx->lds_cap, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(x, t.x.p, sizeof(float) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return collect_timing(ctx);
+}
+
+int mtr_op_bp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol, const double *xc,
+              const double *yc, const float *r, float *x, float lambda)
+{
+    return op_bp_common(ctx, prj, vol, xc, yc, r, x, lambda, false);
+}
+
+int mtr_op_bpt(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol, const double *xc,
+               const double *yc, const float *p, float *x)
+{
+    return op_bp_common(ctx, prj, vol, xc, yc, p, x, 1.0f, true);
+}
+
+int mtr_divide_slice(int myid, int nparts, int Z, int Zstart, mtr_slice *out)
+{
+    if (!out || nparts <= 0 || Z < 0 || myid < 0 || myid >= nparts) return fail(MTR_ERR_ARG, "bad partition");
+    // First Z % nparts parts carry one extra slice (MultiAxis.cpp:13-31).
+    const int rem = Z % nparts, base = Z / nparts;
+    const int h = base + (myid < rem ? 1 : 0);
+    out->z_start = Zstart + myid * base + std::min(myid, rem);
+    out->z_end = out->z_start + h;
+    return MTR_OK;
+}
+
+int mtr_timing_enable(mtr_ctx *ctx, int on)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    MTR_TRY(collect_timing(ctx));
+    ctx->timing = on != 0;
+    return MTR_OK;
+}
+
+int mtr_timing_reset(mtr_ctx *ctx)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    MTR_TRY(collect_timing(ctx));
+    for (int k = 0; k < MTR_K_COUNT; k++) {
+        ctx->launches[k] = 0;
+        ctx->ms[k] = 0;
+    }
+    return MTR_OK;
+}
+
+int mtr_timing_get(mtr_ctx *ctx, int kernel, int64_t *launches, double *ms)
+{
+    if (!ctx || !launches || !ms) return fail(MTR_ERR_ARG, "null argument");
+    if (kernel < 0 || kernel >= MTR_K_COUNT) return fail(MTR_ERR_RANGE, "unknown kernel class %d", kernel);
+    MTR_TRY(collect_timing(ctx));
+    *launches = ctx->launches[kernel];
+    *ms = ctx->ms[kernel];
+    return MTR_OK;
+}
+
+int mtr_sync(mtr_ctx *ctx)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    MTR_TRY(set_device(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return collect_timing(ctx);
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+// Internal types shared by the HIP kernels and the host-side context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "mtr.h"
+
+namespace mtr {
+
+constexpr int kMaxAgents = 16;  // MAX_TILT, ReadAndInitialOptions.h:17
+
+// Per-view geometry record in HBM, read through scalar loads.
+// c[0..3] = xc[4a..4a+3], c[4..7] = yc[4a..4a+3] (the reference's
+// coefficient pairs, FunctionOnGPU.cu:17-18).
+struct ViewCoef {
+    double c[8];
+};
+
+// Ray-driven forward-projection plan of one view.  A view is
+// "row-separable" when v depends on exactly one voxel axis W with
+// coefficient +-1 and an integer offset: detector row k then collects
+// exactly the voxels of plane W(k) (weight 1) and the forward projection of
+// the row is a 2-D gather inside that plane.  Other views use the general
+// 3-D plan: planes perpendicular to the driving axis D (largest component
+// of the ray direction), and inside each plane the bounding box of the
+// support parallelogram |u - i| < 1, |v - k| < 1.
+struct FpPlan {
+    int kind;      // 0 = general, 1 = row-separable
+    int D, P, Q;   // general: driving axis and in-plane axes (0=X,1=Y,2=Z)
+    int W, L, M;   // separable: plane axis, line (drive) axis, cross axis
+    int wsign;     // separable: bW (+1/-1)
+    double woff;   // separable: W = (k + 1 - yc0) * wsign
+    // general: crossing point (p*, q*) = p0 + i*pi + k*pk + d*pd
+    double p0, pi, pk, pd;
+    double q0, qi, qk, qd;
+    double Ep, Eq;  // half extents of the support bounding box
+    // separable: m*(l) = m0 + i*mi + w*mw + l*ml ; half extent Em
+    double m0, mi, mw, ml, Em;
+};
+
+// Device-side description of one agent, passed by value to kernels.
+struct AgentDev {
+    int nu, nv, A;
+    int Xs, Xe, Ys, Ye, Zs, Ze;  // volume box (absolute voxel coordinates)
+    int X, Y, Z;
+    const ViewCoef *coef;  // [A]
+    const FpPlan *plan;    // [A]
+    const void *p;         // projections [A][nv][nu] (storage precision)
+    const float *c;        // FP normaliser [A][nv][nu] (FP32 always)
+    void *r;               // residual [A][nv][nu]
+    void *x;               // working volume [Z][Y][X]
+    double *partial;       // per-block FP64 partial sums
+};
+
+}  // namespace mtr

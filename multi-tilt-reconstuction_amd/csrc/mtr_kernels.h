@@ -1,0 +1,39 @@
+// Launchers of the HIP kernels (host-callable, defined in mtr_kernels.hip).
+#pragma once
+
+#include "mtr_internal.h"
+
+namespace mtr {
+
+int fp_blocks(const AgentDev &ag, bool lanes_k);
+int bp_blocks(const AgentDev &ag, int tz);
+int consensus_blocks(int64_t N);
+
+// Forward projection of nag agents (device table d_ags), mode 0 = fused
+// residual, 1 = raw s/c into s_out/c_out, 2 = normaliser into c_out.
+hipError_t launch_fp(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, bool lanes_k,
+                     float *s_out, float *c_out, hipStream_t st);
+// Back-projection (bpt: initial model) of nag agents; src overrides the
+// agents' residual buffers when non-null (single-agent use).
+hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, int tz, bool bpt,
+                     float lambda, const void *src, int lds_cap, hipStream_t st);
+hipError_t launch_consensus(int prec, int K, int64_t N, float rho, int write_next, void *const *w,
+                            void *const *x, void *wbar, double *partial, int blocks, hipStream_t st);
+hipError_t launch_init_sum(int prec, int K, int64_t N, void *const *w, void *const *x, void *wbar,
+                           hipStream_t st);
+hipError_t launch_form_z(int prec, int64_t N, const void *w, const void *wbar, void *z, hipStream_t st);
+hipError_t launch_mann(int prec, int64_t N, const void *v, void *w, const void *z, float rho,
+                       hipStream_t st);
+hipError_t launch_mann_z(int prec, int64_t N, const void *v, void *w, const void *wbar, float rho,
+                         hipStream_t st);
+hipError_t launch_mean(int prec, int K, int64_t N, const void *const *w, void *wbar, hipStream_t st);
+hipError_t launch_residual(int prec, int64_t P, const void *p, const float *s, const float *c, void *r,
+                           hipStream_t st);
+hipError_t launch_to_storage(int prec, int64_t n, const float *src, void *dst, hipStream_t st);
+hipError_t launch_from_storage(int prec, int64_t n, const void *src, float *dst, hipStream_t st);
+hipError_t launch_fill(int prec, int64_t n, void *dst, float v, hipStream_t st);
+hipError_t launch_reduce(const double *part, int64_t n, double *out, hipStream_t st);
+hipError_t launch_plain_splat(const AgentDev &ag, float *s, float *c, hipStream_t st);
+hipError_t launch_plain_bp(const AgentDev &ag, const float *r, float lambda, int bpt, hipStream_t st);
+
+}  // namespace mtr

@@ -1,0 +1,915 @@
+// HIP kernels of the SIRT / MACE hot path for CDNA4 (gfx950, wave64).
+//
+// Operator (file:line in Autumn1998/Multi-tilt-Reconstuction):
+//   geometry     u = xc0 + xc1 X + xc2 Y + xc3 Z - 1 (FP64)   FunctionOnGPU.cu:12-25
+//   FP (splat)   s = sum fl32(w x), c = sum fl32(w)            FunctionOnGPU.cu:27-52, 316-333
+//   residual     r = p - (c != 0 ? s/c : s)                    FunctionOnGPU.cu:228-236
+//   BP (gather)  x += fl32(S/C) * lambda where C != 0          FunctionOnGPU.cu:54-79, 111-135
+//   BPT          x  = S/C of p                                 FunctionOnGPU.cu:82-109
+//   z            z = 2 wbar - w                                FunctionOnGPU.cu:268-276
+//   Mann         w = rho (2v - z) + (1 - rho) w                MultiAxis.cpp:33-38
+//   mean         wbar = sum_a w_a / K                          MultiAxis.cpp:61-66
+//
+// The forward projection is computed as a ray-driven gather: every
+// detector pixel sums the weights of the voxels in its bilinear support,
+// so no atomics are involved and the result is deterministic.  Each
+// contribution is formed exactly as the reference's splat forms it
+// (same FP64 coordinate chain, same weight products, same FP32 rounding);
+// only the summation order differs, which the reference leaves undefined.
+#include "mtr_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+namespace mtr {
+
+// ------------------------------------------------------------------------
+// storage helpers
+template <class T>
+__device__ __forceinline__ float ldf(const T *p, int64_t i);
+template <>
+__device__ __forceinline__ float ldf<float>(const float *p, int64_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ float ldf<__hip_bfloat16>(const __hip_bfloat16 *p, int64_t i)
+{
+    return __bfloat162float(p[i]);
+}
+template <>
+__device__ __forceinline__ float ldf<__half>(const __half *p, int64_t i) { return __half2float(p[i]); }
+
+template <class T>
+__device__ __forceinline__ void stf(T *p, int64_t i, float v);
+template <>
+__device__ __forceinline__ void stf<float>(float *p, int64_t i, float v) { p[i] = v; }
+template <>
+__device__ __forceinline__ void stf<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i, float v)
+{
+    p[i] = __float2bfloat16(v);
+}
+template <>
+__device__ __forceinline__ void stf<__half>(__half *p, int64_t i, float v) { p[i] = __float2half(v); }
+
+// The reference's coordinate chain, left to right with fused multiply-adds.
+__device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
+{
+    return fma(c[3], (double)Z, fma(c[2], (double)Y, fma(c[1], (double)X, c[0]))) - 1.0;
+}
+
+// Select among three per-axis values without a dynamically indexed array.
+template <class V>
+__device__ __forceinline__ V sel3(V x, V y, V z, int axis)
+{
+    return axis == 0 ? x : (axis == 1 ? y : z);
+}
+
+// Block-wide FP64 sum of one value per thread; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < NT / 64; w++) t += sh[w];
+    return t;
+}
+
+// ------------------------------------------------------------------------
+// Forward projection, ray-driven gather.
+//
+// Block: 64 x 4 pixels of one view of one agent (blockIdx.y).  Thread
+// (i, k) walks the planes of its plan and gathers the voxels of its
+// support, forming each contribution as fl32(w * x) with w the reference's
+// FP64 weight product.
+//
+// MODE 0: residual r = p - (c != 0 ? s / c : s) with the precomputed
+//         normaliser, FP64 sum of r^2 per block into ag.partial.
+// MODE 1: raw s and c into s_out / c_out (stateless operator).
+// MODE 2: normaliser c into c_out (geometry only, x taken as 1).
+struct FpAcc {
+    float s, c;
+};
+
+template <class T, bool UNIT_X>
+__device__ __forceinline__ void fp_take(FpAcc &acc, const T *x, int64_t j, double w)
+{
+    acc.c += (float)w;
+    if (UNIT_X)
+        acc.s += (float)w;
+    else
+        acc.s += (float)(w * (double)ldf<T>(x, j));
+}
+
+template <class T, bool UNIT_X>
+__device__ FpAcc fp_ray_general(const AgentDev &ag, const ViewCoef &vc, const FpPlan &pl, int i,
+                                int k)
+{
+    FpAcc acc{0.f, 0.f};
+    const T *x = reinterpret_cast<const T *>(ag.x);
+    const int Dlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.D), Dhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.D) - 1;
+    const int Plo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.P), Phi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.P) - 1;
+    const int Qlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.Q), Qhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.Q) - 1;
+    const double ps = pl.p0 + i * pl.pi + k * pl.pk;
+    const double qs = pl.q0 + i * pl.qi + k * pl.qk;
+    // Planes where the support box can meet the volume.
+    double dlo = Dlo, dhi = Dhi;
+    if (pl.pd != 0.0) {
+        double t0 = (Plo - pl.Ep - 1.0 - ps) / pl.pd, t1 = (Phi + pl.Ep + 1.0 - ps) / pl.pd;
+        dlo = fmax(dlo, fmin(t0, t1));
+        dhi = fmin(dhi, fmax(t0, t1));
+    } else if (ps < Plo - pl.Ep - 1.0 || ps > Phi + pl.Ep + 1.0) {
+        return acc;
+    }
+    if (pl.qd != 0.0) {
+        double t0 = (Qlo - pl.Eq - 1.0 - qs) / pl.qd, t1 = (Qhi + pl.Eq + 1.0 - qs) / pl.qd;
+        dlo = fmax(dlo, fmin(t0, t1));
+        dhi = fmin(dhi, fmax(t0, t1));
+    } else if (qs < Qlo - pl.Eq - 1.0 || qs > Qhi + pl.Eq + 1.0) {
+        return acc;
+    }
+    if (!(dlo <= dhi)) return acc;
+    const int d0 = max(Dlo, (int)floor(dlo)), d1 = min(Dhi, (int)ceil(dhi));
+    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
+    const int64_t sD = sel3<int64_t>(1, sy, sz, pl.D), sP = sel3<int64_t>(1, sy, sz, pl.P),
+                  sQ = sel3<int64_t>(1, sy, sz, pl.Q);
+    const int64_t base = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz;
+    const double *cu = vc.c, *cv = vc.c + 4;
+    for (int d = d0; d <= d1; d++) {
+        const double pc = ps + d * pl.pd, qc = qs + d * pl.qd;
+        const int p0 = max(Plo, (int)ceil(pc - pl.Ep - 1e-9));
+        const int p1 = min(Phi, (int)floor(pc + pl.Ep + 1e-9));
+        const int q0 = max(Qlo, (int)ceil(qc - pl.Eq - 1e-9));
+        const int q1 = min(Qhi, (int)floor(qc + pl.Eq + 1e-9));
+        for (int p = p0; p <= p1; p++)
+            for (int q = q0; q <= q1; q++) {
+                const int X = pl.D == 0 ? d : (pl.P == 0 ? p : q);
+                const int Y = pl.D == 1 ? d : (pl.P == 1 ? p : q);
+                const int Z = pl.D == 2 ? d : (pl.P == 2 ? p : q);
+                const double u = coord(cu, X, Y, Z);
+                const double v = coord(cv, X, Y, Z);
+                const double tu = u - (double)i, tv = v - (double)k;
+                if (fabs(tu) < 1.0 && fabs(tv) < 1.0) {
+                    // The reference's weight of this tap: (1 - du) when
+                    // floor(u) == i, du = u - floor(u) when floor(u) == i - 1.
+                    const double wu = tu >= 0.0 ? 1.0 - tu : u - (double)(i - 1);
+                    const double wv = tv >= 0.0 ? 1.0 - tv : v - (double)(k - 1);
+                    fp_take<T, UNIT_X>(acc, x, base + d * sD + p * sP + q * sQ, wu * wv);
+                }
+            }
+    }
+    return acc;
+}
+
+template <class T, bool UNIT_X>
+__device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const FpPlan &pl, int i,
+                                  int k)
+{
+    FpAcc acc{0.f, 0.f};
+    const T *x = reinterpret_cast<const T *>(ag.x);
+    const int w = (int)(k * pl.wsign + pl.woff);
+    if (w < sel3(ag.Xs, ag.Ys, ag.Zs, pl.W) || w > sel3(ag.Xe, ag.Ye, ag.Ze, pl.W) - 1) return acc;
+    const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
+    const int Mlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.M), Mhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.M) - 1;
+    const double ms = pl.m0 + i * pl.mi + w * pl.mw;
+    double llo = Llo, lhi = Lhi;
+    if (pl.ml != 0.0) {
+        double t0 = (Mlo - pl.Em - 1.0 - ms) / pl.ml, t1 = (Mhi + pl.Em + 1.0 - ms) / pl.ml;
+        llo = fmax(llo, fmin(t0, t1));
+        lhi = fmin(lhi, fmax(t0, t1));
+    } else if (ms < Mlo - pl.Em - 1.0 || ms > Mhi + pl.Em + 1.0) {
+        return acc;
+    }
+    if (!(llo <= lhi)) return acc;
+    const int l0 = max(Llo, (int)floor(llo)), l1 = min(Lhi, (int)ceil(lhi));
+    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
+    const int64_t sW = sel3<int64_t>(1, sy, sz, pl.W), sL = sel3<int64_t>(1, sy, sz, pl.L),
+                  sM = sel3<int64_t>(1, sy, sz, pl.M);
+    const int64_t base = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz + w * sW;
+    const double *cu = vc.c;
+    for (int l = l0; l <= l1; l++) {
+        const double mc = ms + l * pl.ml;
+        const int m0 = max(Mlo, (int)ceil(mc - pl.Em - 1e-9));
+        const int m1 = min(Mhi, (int)floor(mc + pl.Em + 1e-9));
+        for (int m = m0; m <= m1; m++) {
+            const int X = pl.W == 0 ? w : (pl.L == 0 ? l : m);
+            const int Y = pl.W == 1 ? w : (pl.L == 1 ? l : m);
+            const int Z = pl.W == 2 ? w : (pl.L == 2 ? l : m);
+            const double u = coord(cu, X, Y, Z);
+            const double tu = u - (double)i;
+            if (fabs(tu) < 1.0) {
+                const double wu = tu >= 0.0 ? 1.0 - tu : u - (double)(i - 1);
+                // (1 - dv) == 1 exactly: v is integral on this row.
+                fp_take<T, UNIT_X>(acc, x, base + l * sL + m * sM, wu * 1.0);
+            }
+        }
+    }
+    return acc;
+}
+
+template <class T, int MODE, bool LANES_K>
+__global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, float *__restrict__ s_out,
+                                           float *__restrict__ c_out)
+{
+    const AgentDev &ag = ags[blockIdx.y];
+    // 64 x 4 pixel tiles (LANES_K: 4 x 64, lanes along detector rows).
+    const int TI = LANES_K ? 4 : 64, TK = LANES_K ? 64 : 4;
+    const int nti = (ag.nu + TI - 1) / TI, ntk = (ag.nv + TK - 1) / TK;
+    int b = blockIdx.x;
+    if (b >= nti * ntk * ag.A) return;
+    const int bi = b % nti;
+    b /= nti;
+    const int bk = b % ntk;
+    const int a = b / ntk;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int i = bi * TI + (LANES_K ? wid : lane);
+    const int k = bk * TK + (LANES_K ? lane : wid);
+    const bool active = i < ag.nu && k < ag.nv;
+    FpAcc acc{0.f, 0.f};
+    if (active) {
+        const ViewCoef vc = ag.coef[a];
+        const FpPlan pl = ag.plan[a];
+        if (pl.kind == 1)
+            acc = fp_ray_separable<T, MODE == 2>(ag, vc, pl, i, k);
+        else
+            acc = fp_ray_general<T, MODE == 2>(ag, vc, pl, i, k);
+    }
+    const int64_t n = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.nu + i;
+    if (MODE == 0) {
+        double r2 = 0.0;
+        if (active) {
+            const float cc = ag.c[n];
+            float sv = acc.s;
+            if (cc != 0.0f) sv = sv / cc;
+            const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), n) - sv;
+            stf<T>(reinterpret_cast<T *>(ag.r), n, rv);
+            const double rs = (double)ldf<T>(reinterpret_cast<const T *>(ag.r), n);
+            r2 = rs * rs;
+        }
+        __shared__ double sh[4];
+        const double t = block_sum<256>(r2, sh);
+        if (threadIdx.x == 0) ag.partial[blockIdx.x] = t;
+    } else if (MODE == 1) {
+        if (active) {
+            s_out[n] = acc.s;
+            c_out[n] = acc.c;
+        }
+    } else {
+        if (active) c_out[n] = acc.c;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Back-projection, voxel-driven gather with LDS-staged projection rows.
+//
+// Block: 64 (X) x 4 (Y) voxel columns of TZ voxels each; blockIdx.y is the
+// agent.  For every view the block stages the detector footprint of its
+// tile (rows of the projection, with a one-pixel zero border outside the
+// detector) into LDS and every voxel gathers its four taps from there.
+// Views whose footprint does not fit the LDS budget gather from HBM.
+// BPT: x = S/C (or 0); otherwise x = fl32(x + fl32(S/C) * lambda) where
+// C != 0, with S and C accumulated in FP32 over views in order.
+template <class T, int TZ, bool BPT>
+__global__ __launch_bounds__(256) void k_bp(const AgentDev *__restrict__ ags, float lambda,
+                                           const T *__restrict__ src_override, int lds_cap)
+{
+    extern __shared__ float lds[];
+    const AgentDev &ag = ags[blockIdx.y];
+    const int ntx = (ag.X + 63) / 64, nty = (ag.Y + 3) / 4, ntz = (ag.Z + TZ - 1) / TZ;
+    int b = blockIdx.x;
+    if (b >= ntx * nty * ntz) return;
+    const int bx = b % ntx;
+    b /= ntx;
+    const int by = b % nty;
+    const int bz = b / nty;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int X0 = ag.Xs + bx * 64, Y0 = ag.Ys + by * 4, Z0 = ag.Zs + bz * TZ;
+    const int X = X0 + tx, Y = Y0 + ty;
+    const bool active = X < ag.Xe && Y < ag.Ye;
+    const int ex = min(64, ag.Xe - X0) - 1, ey = min(4, ag.Ye - Y0) - 1, ez = min(TZ, ag.Ze - Z0) - 1;
+    const int nu = ag.nu, nv = ag.nv;
+    const int64_t plane = (int64_t)nu * nv;
+    const T *rsrc = src_override ? src_override : reinterpret_cast<const T *>(ag.r);
+
+    float S[TZ], C[TZ];
+#pragma unroll
+    for (int t = 0; t < TZ; t++) S[t] = C[t] = 0.f;
+
+    for (int a = 0; a < ag.A; a++) {
+        const ViewCoef vc = ag.coef[a];
+        const double *cu = vc.c, *cv = vc.c + 4;
+        // Footprint of the tile (affine map: extremes at the corners).
+        const double u0 = coord(cu, X0, Y0, Z0), v0 = coord(cv, X0, Y0, Z0);
+        const double ulo = u0 + fmin(0.0, cu[1] * ex) + fmin(0.0, cu[2] * ey) + fmin(0.0, cu[3] * ez);
+        const double uhi = u0 + fmax(0.0, cu[1] * ex) + fmax(0.0, cu[2] * ey) + fmax(0.0, cu[3] * ez);
+        const double vlo = v0 + fmin(0.0, cv[1] * ex) + fmin(0.0, cv[2] * ey) + fmin(0.0, cv[3] * ez);
+        const double vhi = v0 + fmax(0.0, cv[1] * ex) + fmax(0.0, cv[2] * ey) + fmax(0.0, cv[3] * ez);
+        const int fi0 = max((int)floor(ulo) - 1, -1), fi1 = min((int)floor(uhi) + 2, nu);
+        const int fk0 = max((int)floor(vlo) - 1, -1), fk1 = min((int)floor(vhi) + 2, nv);
+        if (fi1 < 0 || fi0 > nu - 1 || fk1 < 0 || fk0 > nv - 1) continue;  // view misses the tile
+        const int W = fi1 - fi0 + 1, H = fk1 - fk0 + 1;
+        const bool interior = fi0 >= 0 && fi1 <= nu - 1 && fk0 >= 0 && fk1 <= nv - 1;
+        const bool staged = W * H <= lds_cap;
+        const T *rv = rsrc + plane * a;
+        if (staged) {
+            __syncthreads();
+            for (int row = ty; row < H; row += 4) {
+                const int gk = fk0 + row;
+                const bool rok = gk >= 0 && gk < nv;
+                for (int col = tx; col < W; col += 64) {
+                    const int gi = fi0 + col;
+                    float val = 0.f;
+                    if (rok && gi >= 0 && gi < nu) val = ldf<T>(rv, (int64_t)gk * nu + gi);
+                    lds[row * W + col] = val;
+                }
+            }
+            __syncthreads();
+        }
+        double u = coord(cu, X, Y, Z0), v = coord(cv, X, Y, Z0);
+#pragma unroll
+        for (int t = 0; t < TZ; t++) {
+            const double fu = floor(u), fv = floor(v);
+            const int i0 = (int)fu, k0 = (int)fv;
+            const float du = (float)(u - fu), dv = (float)(v - fv);
+            float r00, r01, r10, r11;
+            if (staged) {
+                const int li = min(max(i0 - fi0, 0), W - 2), lk = min(max(k0 - fk0, 0), H - 2);
+                const float *q = lds + lk * W + li;
+                r00 = q[0];
+                r01 = q[1];
+                r10 = q[W];
+                r11 = q[W + 1];
+            } else {
+                const int ia = min(max(i0, 0), nu - 1), ib = min(max(i0 + 1, 0), nu - 1);
+                const int ka = min(max(k0, 0), nv - 1), kb = min(max(k0 + 1, 0), nv - 1);
+                r00 = ldf<T>(rv, (int64_t)ka * nu + ia);
+                r01 = ldf<T>(rv, (int64_t)ka * nu + ib);
+                r10 = ldf<T>(rv, (int64_t)kb * nu + ia);
+                r11 = ldf<T>(rv, (int64_t)kb * nu + ib);
+            }
+            if (interior) {
+                const float top = fmaf(du, r01 - r00, r00);
+                const float bot = fmaf(du, r11 - r10, r10);
+                S[t] += fmaf(dv, bot - top, top);
+                C[t] += 1.0f;
+            } else {
+                const float wu0 = (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
+                const float wu1 = (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
+                const float wv0 = (unsigned)k0 < (unsigned)nv ? 1.f - dv : 0.f;
+                const float wv1 = (unsigned)(k0 + 1) < (unsigned)nv ? dv : 0.f;
+                S[t] += wv0 * (wu0 * r00 + wu1 * r01) + wv1 * (wu0 * r10 + wu1 * r11);
+                C[t] += (wu0 + wu1) * (wv0 + wv1);
+            }
+            u += cu[3];
+            v += cv[3];
+        }
+    }
+    if (!active) return;
+    T *x = reinterpret_cast<T *>(ag.x);
+    const int64_t sxy = (int64_t)ag.X * ag.Y;
+    const int64_t j0 = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X + (int64_t)(Z0 - ag.Zs) * sxy;
+#pragma unroll
+    for (int t = 0; t < TZ; t++) {
+        if (t > ez) break;
+        const int64_t j = j0 + t * sxy;
+        if (BPT) {
+            stf<T>(x, j, C[t] != 0.0f ? S[t] / C[t] : 0.0f);
+        } else if (C[t] != 0.0f) {
+            stf<T>(x, j, fmaf(S[t] / C[t], lambda, ldf<T>(x, j)));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Fused consensus step over all K agents of a solve (element-wise, HBM
+// bound):  z_a = 2 wbar - w_a (the state this step started from);
+// w_a = rho (2 v_a - z_a) + (1 - rho) w_a; wbar' = sum_a w_a / K;
+// next working volume x_a = 2 wbar' - w_a; FP64 sum of (w_a - wbar')^2.
+// Arithmetic follows the reference's host loops exactly (no contraction).
+struct ConsArgs {
+    int K;
+    int64_t N;
+    float rho;
+    int write_next;
+    void *w[kMaxAgents];
+    void *x[kMaxAgents];
+    void *wbar;
+    double *partial;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_consensus(ConsArgs ca)
+{
+#pragma clang fp contract(off)
+    double d2 = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ca.N; j += stride) {
+        const float wb = ldf<T>(reinterpret_cast<const T *>(ca.wbar), j);
+        float wn[kMaxAgents];
+        float acc = 0.f;
+#pragma unroll
+        for (int a = 0; a < kMaxAgents; a++) {
+            if (a >= ca.K) break;
+            const float wo = ldf<T>(reinterpret_cast<const T *>(ca.w[a]), j);
+            const float vv = ldf<T>(reinterpret_cast<const T *>(ca.x[a]), j);
+            const float z = 2.0f * wb - wo;
+            const float t1 = ca.rho * (2.0f * vv - z);
+            const float t2 = (1.0f - ca.rho) * wo;
+            // Round w_a to storage before it enters the mean, as a stored
+            // state would be.
+            stf<T>(reinterpret_cast<T *>(ca.w[a]), j, t1 + t2);
+            wn[a] = ldf<T>(reinterpret_cast<const T *>(ca.w[a]), j);
+            acc = (a == 0) ? wn[a] : acc + wn[a];
+        }
+        const float wbn_f = acc / (float)ca.K;
+        stf<T>(reinterpret_cast<T *>(ca.wbar), j, wbn_f);
+        const float wbn = ldf<T>(reinterpret_cast<const T *>(ca.wbar), j);
+#pragma unroll
+        for (int a = 0; a < kMaxAgents; a++) {
+            if (a >= ca.K) break;
+            if (ca.write_next) stf<T>(reinterpret_cast<T *>(ca.x[a]), j, 2.0f * wbn - wn[a]);
+            const double e = (double)wn[a] - (double)wbn;
+            d2 += e * e;
+        }
+    }
+    __shared__ double sh[4];
+    const double t = block_sum<256>(d2, sh);
+    if (threadIdx.x == 0) ca.partial[blockIdx.x] = t;
+}
+
+// ------------------------------------------------------------------------
+// Stand-alone element-wise kernels (single agent, the reference's
+// per-call structure).
+template <class T>
+__global__ void k_form_z(int64_t N, const T *__restrict__ w, const T *__restrict__ wbar,
+                         T *__restrict__ z)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride)
+        stf<T>(z, j, 2.0f * ldf<T>(wbar, j) - ldf<T>(w, j));
+}
+
+template <class T>
+__global__ void k_mann(int64_t N, const T *__restrict__ v, T *__restrict__ w, const T *__restrict__ z,
+                       float rho)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+        const float t1 = rho * (2.0f * ldf<T>(v, j) - ldf<T>(z, j));
+        const float t2 = (1.0f - rho) * ldf<T>(w, j);
+        stf<T>(w, j, t1 + t2);
+    }
+}
+
+// Mann update with z recomputed from the step's starting state:
+// w = rho (2v - (2 wbar - w)) + (1 - rho) w.
+template <class T>
+__global__ void k_mann_z(int64_t N, const T *__restrict__ v, T *__restrict__ w,
+                         const T *__restrict__ wbar, float rho)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += stride) {
+        const float wo = ldf<T>(w, j);
+        const float z = 2.0f * ldf<T>(wbar, j) - wo;
+        const float t1 = rho * (2.0f * ldf<T>(v, j) - z);
+        const float t2 = (1.0f - rho) * wo;
+        stf<T>(w, j, t1 + t2);
+    }
+}
+
+// MACE initialisation: B = sum_a BPT_a (agent order, FP32, as the SUM
+// reduction of MultiAxis.cpp:192 with the oracle's order), then
+// w_a = wbar = B and the first working volumes z_a = 2B - B = B.
+struct InitArgs {
+    int K;
+    int64_t N;
+    void *w[kMaxAgents];
+    void *x[kMaxAgents];
+    void *wbar;
+};
+
+template <class T>
+__global__ void k_init_sum(InitArgs ia)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ia.N; j += stride) {
+        float acc = 0.0f;
+        for (int a = 0; a < ia.K; a++) acc += ldf<T>(reinterpret_cast<const T *>(ia.x[a]), j);
+        stf<T>(reinterpret_cast<T *>(ia.wbar), j, acc);
+        const float b = ldf<T>(reinterpret_cast<const T *>(ia.wbar), j);
+        for (int a = 0; a < ia.K; a++) {
+            stf<T>(reinterpret_cast<T *>(ia.w[a]), j, b);
+            stf<T>(reinterpret_cast<T *>(ia.x[a]), j, 2.0f * b - b);
+        }
+    }
+}
+
+struct MeanArgs {
+    int K;
+    int64_t N;
+    const void *w[kMaxAgents];
+    void *wbar;
+};
+
+template <class T>
+__global__ void k_mean(MeanArgs ma)
+{
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ma.N; j += stride) {
+        float acc = ldf<T>(reinterpret_cast<const T *>(ma.w[0]), j);
+        for (int a = 1; a < ma.K; a++) acc += ldf<T>(reinterpret_cast<const T *>(ma.w[a]), j);
+        stf<T>(reinterpret_cast<T *>(ma.wbar), j, acc / (float)ma.K);
+    }
+}
+
+template <class T>
+__global__ void k_residual(int64_t P, const T *__restrict__ p, const float *__restrict__ s,
+                           const float *__restrict__ c, T *__restrict__ r)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < P; j += stride) {
+        float sv = s[j];
+        if (c[j] != 0.0f) sv = sv / c[j];
+        stf<T>(r, j, ldf<T>(p, j) - sv);
+    }
+}
+
+// Storage conversion (float -> T and T -> float).
+template <class T>
+__global__ void k_to_storage(int64_t n, const float *__restrict__ src, T *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+        stf<T>(dst, j, src[j]);
+}
+template <class T>
+__global__ void k_from_storage(int64_t n, const T *__restrict__ src, float *__restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+        dst[j] = ldf<T>(src, j);
+}
+template <class T>
+__global__ void k_fill(int64_t n, T *__restrict__ dst, float v)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+        stf<T>(dst, j, v);
+}
+
+// Deterministic fixed-order sum of n FP64 partials (one block of 256).
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int64_t n,
+                                               double *__restrict__ out)
+{
+    double v = 0.0;
+    for (int64_t j = threadIdx.x; j < n; j += 256) v += part[j];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// ------------------------------------------------------------------------
+// Plain form: the reference's kernel structure on this GPU.  One thread
+// per voxel in 8x8x8 blocks; the forward projection splats with float
+// atomics into zeroed s and c (recomputed every iteration), and the
+// back-projection gathers straight from HBM accumulating S and C in FP64.
+__global__ __launch_bounds__(512) void k_plain_splat(AgentDev ag, float *__restrict__ s,
+                                                    float *__restrict__ c)
+{
+    const int X = ag.Xs + blockIdx.x * 8 + threadIdx.x;
+    const int Y = ag.Ys + blockIdx.y * 8 + threadIdx.y;
+    const int Z = ag.Zs + blockIdx.z * 8 + threadIdx.z;
+    if (X >= ag.Xe || Y >= ag.Ye || Z >= ag.Ze) return;
+    const float *x = reinterpret_cast<const float *>(ag.x);
+    const double xv = (double)x[(int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X +
+                                (int64_t)(Z - ag.Zs) * ag.X * ag.Y];
+    const int nu = ag.nu, nv = ag.nv;
+    for (int a = 0; a < ag.A; a++) {
+        const ViewCoef vc = ag.coef[a];
+        const double u = coord(vc.c, X, Y, Z), v = coord(vc.c + 4, X, Y, Z);
+        const double fu = floor(u), fv = floor(v);
+        const int i0 = (int)fu, k0 = (int)fv;
+        const double du = u - fu, dv = v - fv;
+        const int64_t o = (int64_t)a * nu * nv;
+        const double w[4] = {(1 - du) * (1 - dv), du * (1 - dv), (1 - du) * dv, du * dv};
+        const int ii[4] = {i0, i0 + 1, i0, i0 + 1};
+        const int kk[4] = {k0, k0, k0 + 1, k0 + 1};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if ((unsigned)ii[t] < (unsigned)nu && (unsigned)kk[t] < (unsigned)nv) {
+                const int64_t n = o + (int64_t)kk[t] * nu + ii[t];
+                atomicAdd(&s[n], (float)(w[t] * xv));
+                atomicAdd(&c[n], (float)w[t]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(512) void k_plain_bp(AgentDev ag, const float *__restrict__ r,
+                                                 float lambda, int bpt)
+{
+    const int X = ag.Xs + blockIdx.x * 8 + threadIdx.x;
+    const int Y = ag.Ys + blockIdx.y * 8 + threadIdx.y;
+    const int Z = ag.Zs + blockIdx.z * 8 + threadIdx.z;
+    if (X >= ag.Xe || Y >= ag.Ye || Z >= ag.Ze) return;
+    const int nu = ag.nu, nv = ag.nv;
+    double S = 0.0, C = 0.0;
+    for (int a = 0; a < ag.A; a++) {
+        const ViewCoef vc = ag.coef[a];
+        const double u = coord(vc.c, X, Y, Z), v = coord(vc.c + 4, X, Y, Z);
+        const double fu = floor(u), fv = floor(v);
+        const int i0 = (int)fu, k0 = (int)fv;
+        const double du = u - fu, dv = v - fv;
+        const float *rv = r + (int64_t)a * nu * nv;
+        const double w[4] = {(1 - du) * (1 - dv), du * (1 - dv), (1 - du) * dv, du * dv};
+        const int ii[4] = {i0, i0 + 1, i0, i0 + 1};
+        const int kk[4] = {k0, k0, k0 + 1, k0 + 1};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if ((unsigned)ii[t] < (unsigned)nu && (unsigned)kk[t] < (unsigned)nv) {
+                S = fma(w[t], (double)rv[(int64_t)kk[t] * nu + ii[t]], S);
+                C += w[t];
+            }
+        }
+    }
+    float *x = reinterpret_cast<float *>(ag.x);
+    const int64_t j = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X + (int64_t)(Z - ag.Zs) * ag.X * ag.Y;
+    if (bpt) {
+        x[j] = C != 0.0 ? (float)(S / C) : 0.0f;
+    } else if (C != 0.0) {
+        x[j] = (float)fma((double)(float)(S / C), (double)lambda, (double)x[j]);
+    }
+}
+
+// ========================================================================
+// launchers
+namespace {
+inline int ew_grid(int64_t n)
+{
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+}  // namespace
+
+int fp_blocks(const AgentDev &ag, bool lanes_k)
+{
+    const int TI = lanes_k ? 4 : 64, TK = lanes_k ? 64 : 4;
+    return ((ag.nu + TI - 1) / TI) * ((ag.nv + TK - 1) / TK) * ag.A;
+}
+
+int bp_blocks(const AgentDev &ag, int tz)
+{
+    return ((ag.X + 63) / 64) * ((ag.Y + 3) / 4) * ((ag.Z + tz - 1) / tz);
+}
+
+template <class T>
+static hipError_t launch_fp_t(const AgentDev *d_ags, int nag, int max_blocks, int mode, bool lanes_k,
+                              float *s_out, float *c_out, hipStream_t st)
+{
+    dim3 grid(max_blocks, nag), block(256);
+#define MTR_FP(M, LK) hipLaunchKernelGGL((k_fp<T, M, LK>), grid, block, 0, st, d_ags, s_out, c_out)
+    if (lanes_k) {
+        if (mode == 0) MTR_FP(0, true);
+        else if (mode == 1) MTR_FP(1, true);
+        else MTR_FP(2, true);
+    } else {
+        if (mode == 0) MTR_FP(0, false);
+        else if (mode == 1) MTR_FP(1, false);
+        else MTR_FP(2, false);
+    }
+#undef MTR_FP
+    return hipGetLastError();
+}
+
+hipError_t launch_fp(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, bool lanes_k,
+                     float *s_out, float *c_out, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        return launch_fp_t<__hip_bfloat16>(d_ags, nag, max_blocks, mode, lanes_k, s_out, c_out, st);
+    if (prec == MTR_PREC_FP16)
+        return launch_fp_t<__half>(d_ags, nag, max_blocks, mode, lanes_k, s_out, c_out, st);
+    return launch_fp_t<float>(d_ags, nag, max_blocks, mode, lanes_k, s_out, c_out, st);
+}
+
+template <class T>
+static hipError_t launch_bp_t(const AgentDev *d_ags, int nag, int max_blocks, int tz, bool bpt,
+                              float lambda, const void *src, int lds_cap, hipStream_t st)
+{
+    dim3 grid(max_blocks, nag), block(256);
+    const size_t shm = (size_t)lds_cap * sizeof(float);
+    const T *s = reinterpret_cast<const T *>(src);
+#define MTR_BP(TZ, B) hipLaunchKernelGGL((k_bp<T, TZ, B>), grid, block, shm, st, d_ags, lambda, s, lds_cap)
+    if (tz == 4) {
+        if (bpt) MTR_BP(4, true);
+        else MTR_BP(4, false);
+    } else if (tz == 16) {
+        if (bpt) MTR_BP(16, true);
+        else MTR_BP(16, false);
+    } else {
+        if (bpt) MTR_BP(8, true);
+        else MTR_BP(8, false);
+    }
+#undef MTR_BP
+    return hipGetLastError();
+}
+
+hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, int tz, bool bpt,
+                     float lambda, const void *src, int lds_cap, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        return launch_bp_t<__hip_bfloat16>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
+    if (prec == MTR_PREC_FP16)
+        return launch_bp_t<__half>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
+    return launch_bp_t<float>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
+}
+
+int consensus_blocks(int64_t N) { return ew_grid(N) > 8192 ? 8192 : ew_grid(N); }
+
+hipError_t launch_consensus(int prec, int K, int64_t N, float rho, int write_next, void *const *w,
+                            void *const *x, void *wbar, double *partial, int blocks, hipStream_t st)
+{
+    ConsArgs ca;
+    ca.K = K;
+    ca.N = N;
+    ca.rho = rho;
+    ca.write_next = write_next;
+    for (int a = 0; a < kMaxAgents; a++) {
+        ca.w[a] = a < K ? w[a] : nullptr;
+        ca.x[a] = a < K ? x[a] : nullptr;
+    }
+    ca.wbar = wbar;
+    ca.partial = partial;
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_consensus<__hip_bfloat16>, dim3(blocks), dim3(256), 0, st, ca);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_consensus<__half>, dim3(blocks), dim3(256), 0, st, ca);
+    else
+        hipLaunchKernelGGL(k_consensus<float>, dim3(blocks), dim3(256), 0, st, ca);
+    return hipGetLastError();
+}
+
+#define MTR_DISPATCH(prec, KERNEL, ...)                                                     \
+    do {                                                                                    \
+        if ((prec) == MTR_PREC_BF16)                                                        \
+            hipLaunchKernelGGL(KERNEL<__hip_bfloat16>, __VA_ARGS__);                        \
+        else if ((prec) == MTR_PREC_FP16)                                                   \
+            hipLaunchKernelGGL(KERNEL<__half>, __VA_ARGS__);                                \
+        else                                                                                \
+            hipLaunchKernelGGL(KERNEL<float>, __VA_ARGS__);                                 \
+    } while (0)
+
+hipError_t launch_form_z(int prec, int64_t N, const void *w, const void *wbar, void *z, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_form_z<__hip_bfloat16>, dim3(ew_grid(N)), dim3(256), 0, st, N,
+                           (const __hip_bfloat16 *)w, (const __hip_bfloat16 *)wbar, (__hip_bfloat16 *)z);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_form_z<__half>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const __half *)w,
+                           (const __half *)wbar, (__half *)z);
+    else
+        hipLaunchKernelGGL(k_form_z<float>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const float *)w,
+                           (const float *)wbar, (float *)z);
+    return hipGetLastError();
+}
+
+hipError_t launch_mann(int prec, int64_t N, const void *v, void *w, const void *z, float rho,
+                       hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_mann<__hip_bfloat16>, dim3(ew_grid(N)), dim3(256), 0, st, N,
+                           (const __hip_bfloat16 *)v, (__hip_bfloat16 *)w, (const __hip_bfloat16 *)z, rho);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_mann<__half>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const __half *)v,
+                           (__half *)w, (const __half *)z, rho);
+    else
+        hipLaunchKernelGGL(k_mann<float>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const float *)v,
+                           (float *)w, (const float *)z, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_mann_z(int prec, int64_t N, const void *v, void *w, const void *wbar, float rho,
+                         hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_mann_z<__hip_bfloat16>, dim3(ew_grid(N)), dim3(256), 0, st, N,
+                           (const __hip_bfloat16 *)v, (__hip_bfloat16 *)w, (const __hip_bfloat16 *)wbar, rho);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_mann_z<__half>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const __half *)v,
+                           (__half *)w, (const __half *)wbar, rho);
+    else
+        hipLaunchKernelGGL(k_mann_z<float>, dim3(ew_grid(N)), dim3(256), 0, st, N, (const float *)v,
+                           (float *)w, (const float *)wbar, rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_sum(int prec, int K, int64_t N, void *const *w, void *const *x, void *wbar,
+                           hipStream_t st)
+{
+    InitArgs ia;
+    ia.K = K;
+    ia.N = N;
+    for (int a = 0; a < kMaxAgents; a++) {
+        ia.w[a] = a < K ? w[a] : nullptr;
+        ia.x[a] = a < K ? x[a] : nullptr;
+    }
+    ia.wbar = wbar;
+    MTR_DISPATCH(prec, k_init_sum, dim3(ew_grid(N)), dim3(256), 0, st, ia);
+    return hipGetLastError();
+}
+
+hipError_t launch_mean(int prec, int K, int64_t N, const void *const *w, void *wbar, hipStream_t st)
+{
+    MeanArgs ma;
+    ma.K = K;
+    ma.N = N;
+    for (int a = 0; a < kMaxAgents; a++) ma.w[a] = a < K ? w[a] : nullptr;
+    ma.wbar = wbar;
+    MTR_DISPATCH(prec, k_mean, dim3(ew_grid(N)), dim3(256), 0, st, ma);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(int prec, int64_t P, const void *p, const float *s, const float *c, void *r,
+                           hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_residual<__hip_bfloat16>, dim3(ew_grid(P)), dim3(256), 0, st, P,
+                           (const __hip_bfloat16 *)p, s, c, (__hip_bfloat16 *)r);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_residual<__half>, dim3(ew_grid(P)), dim3(256), 0, st, P, (const __half *)p,
+                           s, c, (__half *)r);
+    else
+        hipLaunchKernelGGL(k_residual<float>, dim3(ew_grid(P)), dim3(256), 0, st, P, (const float *)p, s,
+                           c, (float *)r);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_storage(int prec, int64_t n, const float *src, void *dst, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_to_storage<__hip_bfloat16>, dim3(ew_grid(n)), dim3(256), 0, st, n, src,
+                           (__hip_bfloat16 *)dst);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_to_storage<__half>, dim3(ew_grid(n)), dim3(256), 0, st, n, src, (__half *)dst);
+    else
+        hipLaunchKernelGGL(k_to_storage<float>, dim3(ew_grid(n)), dim3(256), 0, st, n, src, (float *)dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_storage(int prec, int64_t n, const void *src, float *dst, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_from_storage<__hip_bfloat16>, dim3(ew_grid(n)), dim3(256), 0, st, n,
+                           (const __hip_bfloat16 *)src, dst);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_from_storage<__half>, dim3(ew_grid(n)), dim3(256), 0, st, n, (const __half *)src,
+                           dst);
+    else
+        hipLaunchKernelGGL(k_from_storage<float>, dim3(ew_grid(n)), dim3(256), 0, st, n, (const float *)src,
+                           dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(int prec, int64_t n, void *dst, float v, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_fill<__hip_bfloat16>, dim3(ew_grid(n)), dim3(256), 0, st, n, (__hip_bfloat16 *)dst, v);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_fill<__half>, dim3(ew_grid(n)), dim3(256), 0, st, n, (__half *)dst, v);
+    else
+        hipLaunchKernelGGL(k_fill<float>, dim3(ew_grid(n)), dim3(256), 0, st, n, (float *)dst, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double *part, int64_t n, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, part, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_splat(const AgentDev &ag, float *s, float *c, hipStream_t st)
+{
+    dim3 block(8, 8, 8), grid((ag.X + 7) / 8, (ag.Y + 7) / 8, (ag.Z + 7) / 8);
+    hipLaunchKernelGGL(k_plain_splat, grid, block, 0, st, ag, s, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_bp(const AgentDev &ag, const float *r, float lambda, int bpt, hipStream_t st)
+{
+    dim3 block(8, 8, 8), grid((ag.X + 7) / 8, (ag.Y + 7) / 8, (ag.Z + 7) / 8);
+    hipLaunchKernelGGL(k_plain_bp, grid, block, 0, st, ag, r, lambda, bpt);
+    return hipGetLastError();
+}
+
+}  // namespace mtr

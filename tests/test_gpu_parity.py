@@ -1,0 +1,278 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Floating-point tolerances (relative L2 = ||gpu - oracle|| / ||oracle||):
+  single operators (FP, BP, BPT)  <= 2e-6   (FP32 summation order only)
+  SIRT / MACE reconstructions     <= 1e-5   (the FP32-mode bar)
+Element-wise MACE kernels and the residual are bit-exact.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+OP_TOL = 2e-6
+SOLVE_TOL = 1e-5
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def problem(mtr, phis, kind=None, X=24, Y=20, Z=12, seed=5, views=9, nu=None, nv=None, origin=(0, 0, 0)):
+    kind = mtr.synth.GEOM_SLAB if kind is None else kind
+    pb = mtr.synth.make_problem("t", X, Y, Z, 8, list(phis), kind=kind, seed=seed, amin=2, amax=6,
+                                theta=np.linspace(-60, 60, views), nu=nu, nv=nv)
+    if origin != (0, 0, 0):
+        # same data, shifted box: the geometry constant absorbs the shift
+        xs, ys, zs = origin
+        pb.vol = mtr.volume_size(X, Y, Z, xs, ys, zs)
+        for a in range(pb.K):
+            for v in range(pb.prjs[a].AngN):
+                pb.xcs[a][4 * v] -= pb.xcs[a][4 * v + 1] * xs + pb.xcs[a][4 * v + 2] * ys + pb.xcs[a][4 * v + 3] * zs
+                pb.ycs[a][4 * v] -= pb.ycs[a][4 * v + 1] * xs + pb.ycs[a][4 * v + 2] * ys + pb.ycs[a][4 * v + 3] * zs
+    return pb
+
+
+GEOMS = [
+    ("axisZ", dict(phis=[0.0], kind=1)),
+    ("slab0", dict(phis=[0.0])),
+    ("slab90", dict(phis=[90.0])),
+    ("slab45", dict(phis=[45.0])),
+    ("slab120", dict(phis=[120.0])),
+    ("ragged", dict(phis=[30.0], X=37, Y=23, Z=11, nu=29, nv=41)),
+    ("origin", dict(phis=[90.0], origin=(5, -3, 2))),
+    ("small_det", dict(phis=[0.0], nu=15, nv=9)),
+]
+
+
+def random_general(mtr, rng, A, nu, nv, vol):
+    from test_oracle import random_affine  # noqa: WPS433
+
+    return random_affine(rng, A, nu, nv, vol)
+
+
+@pytest.mark.parametrize("name,kw", GEOMS, ids=[g[0] for g in GEOMS])
+def test_fp_matches_oracle(mtr, orc, ctx, name, kw):
+    pb = problem(mtr, **kw)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.0, 2.0, (pb.vol.Z, pb.vol.Y, pb.vol.X)).astype(np.float32)
+    say("fp", name)
+    s, c = ctx.op_fp(pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], x)
+    s0, c0 = orc.fp(pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], x)
+    assert orc.rel_l2(c, c0) < OP_TOL
+    assert orc.rel_l2(s, s0) < OP_TOL
+    assert np.array_equal(c == 0, c0 == 0)
+
+
+def test_fp_general_affine(mtr, orc, ctx):
+    rng = np.random.default_rng(3)
+    vol = mtr.volume_size(21, 17, 13, 2, -4, 1)
+    prj = mtr.projection_size(23, 19, 6)
+    xc, yc = random_general(mtr, rng, prj.AngN, prj.X, prj.Y, vol)
+    x = rng.uniform(-1, 1, (vol.Z, vol.Y, vol.X)).astype(np.float32)
+    say("fp general")
+    s, c = ctx.op_fp(prj, vol, xc, yc, x)
+    s0, c0 = orc.fp(prj, vol, xc, yc, x)
+    assert orc.rel_l2(c, c0) < OP_TOL
+    assert orc.rel_l2(s, s0) < OP_TOL
+
+
+@pytest.mark.parametrize("name,kw", GEOMS, ids=[g[0] for g in GEOMS])
+def test_bp_and_bpt_match_oracle(mtr, orc, ctx, name, kw):
+    pb = problem(mtr, **kw)
+    prj, vol, xc, yc = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0]
+    rng = np.random.default_rng(2)
+    r = rng.normal(size=(prj.AngN, prj.Y, prj.X)).astype(np.float32)
+    x0 = rng.uniform(0, 1, (vol.Z, vol.Y, vol.X)).astype(np.float32)
+    say("bp", name)
+    xb = ctx.op_bp(prj, vol, xc, yc, r, x0, 0.5)
+    assert orc.rel_l2(xb, orc.bp(prj, vol, xc, yc, r, x0, 0.5)) < OP_TOL
+    say("bpt", name)
+    xt = ctx.op_bpt(prj, vol, xc, yc, pb.ps[0])
+    assert orc.rel_l2(xt, orc.bpt(prj, vol, xc, yc, pb.ps[0])) < OP_TOL
+
+
+def test_bp_general_affine(mtr, orc, ctx):
+    rng = np.random.default_rng(4)
+    vol = mtr.volume_size(70, 9, 19, -7, 3, 0)
+    prj = mtr.projection_size(31, 27, 5)
+    xc, yc = random_general(mtr, rng, prj.AngN, prj.X, prj.Y, vol)
+    r = rng.normal(size=(prj.AngN, prj.Y, prj.X)).astype(np.float32)
+    x0 = rng.normal(size=(vol.Z, vol.Y, vol.X)).astype(np.float32)
+    say("bp general")
+    xb = ctx.op_bp(prj, vol, xc, yc, r, x0, 1.0)
+    assert orc.rel_l2(xb, orc.bp(prj, vol, xc, yc, r, x0, 1.0)) < OP_TOL
+
+
+def test_residual_is_bit_exact(orc, ctx):
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=4099).astype(np.float32)
+    s = rng.normal(size=4099).astype(np.float32)
+    c = rng.uniform(0, 3, 4099).astype(np.float32)
+    c[::7] = 0
+    say("residual")
+    assert np.array_equal(ctx.op_residual(p, s, c), orc.residual(p, s, c))
+
+
+def test_line_golden_on_gpu(mtr, ctx):
+    g = json.load(open(os.path.join(GOLD, "line.json")))
+    vol = mtr.volume_size(*g["vol"])
+    prj = mtr.projection_size(*g["prj"])
+    x = np.array(g["x"], np.float32).reshape(1, 1, 2)
+    say("line golden")
+    s, c = ctx.op_fp(prj, vol, g["xc"], g["yc"], x)
+    assert s.ravel().tolist() == g["s"] and c.ravel().tolist() == g["c"]
+    r = ctx.op_residual(np.array(g["p"], np.float32), s.ravel(), c.ravel())
+    assert r.tolist() == g["r"]
+    xb = ctx.op_bp(prj, vol, g["xc"], g["yc"], r.reshape(s.shape), x, g["lam"])
+    assert xb.ravel().tolist() == g["x_after_sirt1"]
+    xt = ctx.op_bpt(prj, vol, g["xc"], g["yc"], np.array(g["p"], np.float32).reshape(s.shape))
+    assert xt.ravel().tolist() == g["bpt"]
+
+
+def load_ctx(mtr, pb, precision=0):
+    c = mtr.Context(0, precision)
+    c.set_volume(pb.vol)
+    for a in range(pb.K):
+        c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+    return c
+
+
+@pytest.mark.parametrize("phi,kind", [(0.0, 1), (90.0, 0), (45.0, 0)])
+def test_sirt_matches_oracle(mtr, orc, phi, kind):
+    pb = problem(mtr, [phi], kind=kind, X=32, Y=28, Z=16, views=13)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    say("sirt", phi, kind)
+    with load_ctx(mtr, pb) as c:
+        x0 = c.bpt(0)
+        assert orc.rel_l2(x0, orc.bpt(prj, vol, xc, yc, p)) < OP_TOL
+        out = c.sirt(0, 10, 1.0)
+        x9 = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 9, 1.0)
+        ref = orc.sirt(prj, vol, xc, yc, p, x9, 1, 1.0)
+        assert orc.rel_l2(out, ref) < SOLVE_TOL
+        # FP64 residual norm of the last iteration against the oracle's
+        s, cc = orc.fp(prj, vol, xc, yc, x9)
+        r = orc.residual(p, s, cc).astype(np.float64)
+        assert c.residual_norm2(0) == pytest.approx(float((r * r).sum()), rel=1e-4)
+
+
+def test_mace_golden_on_gpu(mtr):
+    g = json.load(open(os.path.join(GOLD, "mace.json")))
+    vol = mtr.volume_size(1, 1, 1)
+    prj = mtr.projection_size(1, 1, 1)
+    xc, yc = [1.0, 1.0, 0.0, 0.0], [1.0, 0.0, 1.0, 0.0]
+    for form in (mtr.FORM_FAST, mtr.FORM_PLAIN):
+        for case in g["cases"]:
+            for M, expect in enumerate(case["wbar"]):
+                say("mace golden", form, case["lam"], M)
+                with mtr.Context(0) as c:
+                    c.set_volume(vol)
+                    for q in g["p"]:
+                        c.add_agent(prj, xc, yc, np.array([[[q]]], np.float32))
+                    out = c.mace_solve(M, g["S"], case["lam"], g["rho"], form=form)
+                    assert float(out.ravel()[0]) == expect
+                    assert float(c.get_volume(mtr.VOL_W, 0).ravel()[0]) == case["w1"][M]
+                    assert float(c.get_volume(mtr.VOL_W, 1).ravel()[0]) == case["w2"][M]
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["fast", "plain"])
+def test_mace_matches_oracle(mtr, orc, form):
+    pb = problem(mtr, [0.0, 90.0, 45.0], X=30, Y=26, Z=14, views=11, seed=8)
+    say("mace", form)
+    with load_ctx(mtr, pb) as c:
+        out = c.mace_solve(3, 4, 0.5, 0.5, form=form)
+    ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 3, 4, 0.5, 0.5)
+    ref64 = orc.mace_f64(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 3, 4, 0.5, 0.5)
+    assert orc.rel_l2(out, ref) < SOLVE_TOL
+    assert orc.rel_l2(out, ref64) < SOLVE_TOL
+
+
+def test_mace_steps_through_api_match_solve(mtr, orc):
+    # the reference-shaped single calls (BPT, z, SIRT, Mann, mean) compose
+    # to the same result as the fused solve
+    pb = problem(mtr, [0.0, 90.0], X=20, Y=20, Z=10, views=7, seed=9)
+    say("mace api")
+    with load_ctx(mtr, pb) as c:
+        fused = c.mace_solve(2, 3, 0.5, 0.5)
+    with load_ctx(mtr, pb) as c:
+        c.mace_init()
+        for _ in range(2):
+            for a in range(pb.K):
+                c.form_z(a)
+                c.sirt(a, 3, 0.5, out=False)
+                c.mann(a, 0.5)
+            c.mean()
+        stepped = c.get_volume(mtr.VOL_WBAR)
+    assert np.array_equal(fused, stepped)
+
+
+def test_elementwise_bit_exact(mtr, orc):
+    pb = problem(mtr, [0.0, 90.0], X=16, Y=12, Z=6, views=5)
+    rng = np.random.default_rng(12)
+    shape = (pb.vol.Z, pb.vol.Y, pb.vol.X)
+    w = [rng.normal(size=shape).astype(np.float32) for _ in range(2)]
+    wb = rng.normal(size=shape).astype(np.float32)
+    v = rng.normal(size=shape).astype(np.float32)
+    say("elementwise")
+    with load_ctx(mtr, pb) as c:
+        for a in range(2):
+            c.set_volume_data(mtr.VOL_W, a, w[a])
+        c.set_volume_data(mtr.VOL_WBAR, 0, wb)
+        c.form_z(0)
+        z = c.get_volume(mtr.VOL_WORK, 0)
+        assert np.array_equal(z, orc.form_z(w[0], wb))
+        c.set_volume_data(mtr.VOL_WORK, 0, v)
+        c.mann(0, 0.3)
+        assert np.array_equal(c.get_volume(mtr.VOL_W, 0), orc.mann(v, w[0], orc.form_z(w[0], wb), 0.3))
+        c.mean()
+        assert np.array_equal(c.get_volume(mtr.VOL_WBAR), orc.mean([c.get_volume(mtr.VOL_W, 0), w[1]]))
+
+
+def test_solve_is_deterministic(mtr):
+    pb = problem(mtr, [0.0, 45.0], X=40, Y=36, Z=12, views=9, seed=13)
+    say("determinism")
+    outs = []
+    for _ in range(2):
+        with load_ctx(mtr, pb) as c:
+            outs.append(c.mace_solve(2, 3, 0.5, 0.5))
+            outs.append(c.consensus_diff2())
+    assert np.array_equal(outs[0], outs[2]) and outs[1] == outs[3]
+
+
+def test_c1_full_size_matches_oracle(mtr, orc):
+    pb = mtr.synth.config("c1")
+    say("c1 full size")
+    with load_ctx(mtr, pb) as c:
+        out = c.mace_solve(pb.M, pb.S, pb.lam, pb.rho)
+    ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, pb.M, pb.S, pb.lam, pb.rho)
+    err = orc.rel_l2(out, ref)
+    say("c1 rel_l2 vs oracle", err)
+    assert err < SOLVE_TOL
+
+
+def test_c2_full_size_one_step_matches_oracle(mtr, orc):
+    pb = mtr.synth.config("c2")
+    say("c2 full size, M=1 S=1")
+    with load_ctx(mtr, pb) as c:
+        out = c.mace_solve(1, 1, pb.lam, pb.rho)
+    ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 1, 1, pb.lam, pb.rho)
+    err = orc.rel_l2(out, ref)
+    say("c2 rel_l2 vs oracle", err)
+    assert err < SOLVE_TOL
+
+
+def test_c2_full_solve_properties(mtr):
+    pb = mtr.synth.config("c2")
+    say("c2 full solve")
+    with load_ctx(mtr, pb) as c:
+        out = c.mace_solve(pb.M, pb.S, pb.lam, pb.rho)
+        d2 = c.consensus_diff2()
+        r2 = [c.residual_norm2(a) for a in range(pb.K)]
+    assert np.isfinite(out).all() and np.isfinite(d2) and all(np.isfinite(r2))
+    assert float(np.abs(out).max()) > 0
