@@ -14,6 +14,9 @@
  *   mtr_sirt           runSIRTOnGPU             FunctionOnGPU.cu:182-266 (declared FunctionOnGPU.cuh:42)
  *   mtr_form_z         initialZdata             FunctionOnGPU.cu:291-313 (declared FunctionOnGPU.cuh:43)
  *   mtr_mann           runMaceOnCPU             MultiAxis.cpp:33-38 (GPU twin runMaceOnGPU FunctionOnGPU.cuh:44)
+ *   mtr_op_form_z      initialZOnGPU            FunctionOnGPU.cu:268-276
+ *   mtr_op_mann        runMaceOnCPU             MultiAxis.cpp:33-38
+ *   mtr_op_mean        consensus average        MultiAxis.cpp:64-65
  *   mtr_mean           consensus average        MultiAxis.cpp:61-66
  *   mtr_mace_init      BPT-sum initialisation   MultiAxis.cpp:163-200
  *   mtr_mace_solve     reconstructModel         MultiAxis.cpp:40-71 (+ the initialisation above)
@@ -174,6 +177,12 @@ int mtr_op_bp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
 /* x = fl32(S/C) of p (x need not be initialised). */
 int mtr_op_bpt(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
                const double *xc, const double *yc, const float *p, float *x);
+/* z = 2*wbar - w (initialZOnGPU, FunctionOnGPU.cu:268-276). */
+int mtr_op_form_z(mtr_ctx *ctx, int64_t n, const float *w, const float *wbar, float *z);
+/* w = rho*(2v - z) + (1-rho)*w in place (runMaceOnCPU, MultiAxis.cpp:33-38). */
+int mtr_op_mann(mtr_ctx *ctx, int64_t n, const float *v, float *w, const float *z, float rho);
+/* wbar = (w[0] + ... + w[K-1]) / K (MultiAxis.cpp:64-65). */
+int mtr_op_mean(mtr_ctx *ctx, int64_t n, int K, const float *const *w, float *wbar);
 /* Z-slab partition of the reference (MultiAxis.cpp:13-31). */
 int mtr_divide_slice(int myid, int nparts, int Z, int Zstart, mtr_slice *out);
 

@@ -25,7 +25,7 @@ import ctypes as _ct
 
 import numpy as _np
 
-from . import synth  # noqa: F401  (re-exported generator helpers)
+from . import mrc, synth  # noqa: F401  (file boundary and generator helpers)
 from ._lib import (  # noqa: F401
     LIB_PATH,
     MtrError,
@@ -231,6 +231,25 @@ class Context:
         r = _np.empty_like(p)
         _check(lib.mtr_op_residual(self._h, p.size, _fp(p), _fp(s), _fp(c), _fp(r)), "op_residual")
         return r
+
+    def op_form_z(self, w, wbar):
+        w, wbar = _f32(w), _f32(wbar)
+        z = _np.empty_like(w)
+        _check(lib.mtr_op_form_z(self._h, w.size, _fp(w), _fp(wbar), _fp(z)), "op_form_z")
+        return z
+
+    def op_mann(self, v, w, z, rho: float):
+        v, z = _f32(v), _f32(z)
+        wo = _f32(w).copy()
+        _check(lib.mtr_op_mann(self._h, wo.size, _fp(v), _fp(wo), _fp(z), _ct.c_float(rho)), "op_mann")
+        return wo
+
+    def op_mean(self, ws):
+        ws = [_f32(w) for w in ws]
+        arr = (_ct.POINTER(_ct.c_float) * len(ws))(*[_fp(w) for w in ws])
+        out = _np.empty_like(ws[0])
+        _check(lib.mtr_op_mean(self._h, ws[0].size, len(ws), arr, _fp(out)), "op_mean")
+        return out
 
     def op_bp(self, prj: ProjectionSize, vol: VolumeSize, xc, yc, r, x, lam: float):
         xc, yc = _f64(xc), _f64(yc)

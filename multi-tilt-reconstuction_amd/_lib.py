@@ -71,10 +71,20 @@ _PROTOS = {
     "mtr_op_bp": (_i, [_v, _pprj, _pvol, _pd, _pd, _pf, _pf, _f]),
     "mtr_op_bpt": (_i, [_v, _pprj, _pvol, _pd, _pd, _pf, _pf]),
     "mtr_divide_slice": (_i, [_i, _i, _i, _i, ct.POINTER(Slice)]),
+    "mtr_op_form_z": (_i, [_v, _i64, _pf, _pf, _pf]),
+    "mtr_op_mann": (_i, [_v, _i64, _pf, _pf, _pf, _f]),
+    "mtr_op_mean": (_i, [_v, _i64, _i, ct.POINTER(_pf), _pf]),
     "mtr_timing_enable": (_i, [_v, _i]),
     "mtr_timing_reset": (_i, [_v]),
     "mtr_timing_get": (_i, [_v, _i, ct.POINTER(_i64), _pd]),
     "mtr_sync": (_i, [_v]),
+    # file boundary (mtr_io.h)
+    "mtr_mrc_read_header": (_i, [ct.c_char_p, _v]),
+    "mtr_mrc_read_data": (_i, [ct.c_char_p, _pf, _i64]),
+    "mtr_mrc_write_volume": (_i, [ct.c_char_p, _pvol, _pf]),
+    "mtr_mrc_write_stack": (_i, [ct.c_char_p, _i, _i, _i, _pf]),
+    "mtr_txbr_read": (_i, [ct.c_char_p, _pvol, _pd, _pd, _i, _pi]),
+    "mtr_txbr_write": (_i, [ct.c_char_p, _pvol, _pd, _pd, _i]),
     # synthetic generator (mtr_synth.h)
     "mtr_synth_geometry": (_i, [_pvol, _pprj, _i, _d, _pd, _pd, _pd]),
     "mtr_synth_phantom": (_i, [ct.c_uint64, _pvol, _i, _d, _d, _d, _d, _pd]),

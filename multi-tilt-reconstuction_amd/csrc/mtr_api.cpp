@@ -941,6 +941,61 @@ int mtr_op_residual(mtr_ctx *ctx, int64_t n, const float *p, const float *s, con
     return MTR_OK;
 }
 
+int mtr_op_form_z(mtr_ctx *ctx, int64_t n, const float *w, const float *wbar, float *z)
+{
+    if (!ctx || !w || !wbar || !z || n < 0) return fail(MTR_ERR_ARG, "bad argument");
+    if (n == 0) return MTR_OK;
+    MTR_TRY(set_device(ctx));
+    DevBuf dw, db, dz;
+    HIP_TRY(hipMalloc(&dw.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&db.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&dz.p, sizeof(float) * n));
+    HIP_TRY(hipMemcpyAsync(dw.p, w, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(db.p, wbar, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_form_z(MTR_PREC_FP32, n, dw.p, db.p, dz.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(z, dz.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int mtr_op_mann(mtr_ctx *ctx, int64_t n, const float *v, float *w, const float *z, float rho)
+{
+    if (!ctx || !v || !w || !z || n < 0) return fail(MTR_ERR_ARG, "bad argument");
+    if (n == 0) return MTR_OK;
+    MTR_TRY(set_device(ctx));
+    DevBuf dv, dw, dz;
+    HIP_TRY(hipMalloc(&dv.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&dw.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&dz.p, sizeof(float) * n));
+    HIP_TRY(hipMemcpyAsync(dv.p, v, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dw.p, w, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dz.p, z, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_mann(MTR_PREC_FP32, n, dv.p, dw.p, dz.p, rho, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(w, dw.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int mtr_op_mean(mtr_ctx *ctx, int64_t n, int K, const float *const *w, float *wbar)
+{
+    if (!ctx || !w || !wbar || n < 0 || K <= 0 || K > kMaxAgents) return fail(MTR_ERR_ARG, "bad argument");
+    if (n == 0) return MTR_OK;
+    MTR_TRY(set_device(ctx));
+    DevBuf dw[kMaxAgents], db;
+    std::vector<const void *> ptrs;
+    for (int a = 0; a < K; a++) {
+        if (!w[a]) return fail(MTR_ERR_ARG, "null agent volume %d", a);
+        HIP_TRY(hipMalloc(&dw[a].p, sizeof(float) * n));
+        HIP_TRY(hipMemcpyAsync(dw[a].p, w[a], sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+        ptrs.push_back(dw[a].p);
+    }
+    HIP_TRY(hipMalloc(&db.p, sizeof(float) * n));
+    HIP_TRY(launch_mean(MTR_PREC_FP32, K, n, ptrs.data(), db.p, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(wbar, db.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
 static int op_bp_common(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
                         const double *xc, const double *yc, const float *r, float *x, float lambda, bool bpt)
 {

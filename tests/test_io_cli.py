@@ -1,0 +1,175 @@
+"""File boundary (MRC, TXBR), the MultiAxis-compatible CLI and the
+reference-shaped host functions.
+
+CPU tests pin the 1024-byte header layout (MrcFileIOBaseFunction.h:99-174),
+round trips, the four pixel modes and the TXBR grammar (readCoef,
+MrcFileIOBaseFunction.cpp:72-134).  GPU tests run the CLI and the host
+functions on tiny synthetic files and compare with the CPU oracle.
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "multi-tilt-reconstuction_amd")
+CLI = os.path.join(PKG, "mtr_multiaxis")
+CHECK = os.path.join(PKG, "mtr_compat_check")
+
+# byte offsets of the packed header (MRC convention of the reference)
+OFFSETS = {"nx": 0, "ny": 4, "nz": 8, "mode": 12, "nxstart": 16, "mx": 28, "xlen": 40, "alpha": 52, "mapc": 64,
+           "amin": 76, "amax": 80, "amean": 84, "ispg": 88, "next": 92, "creatid": 96, "nint": 128, "min2": 136,
+           "idtype": 160, "tiltangles": 172, "xorg": 196, "cmap": 208, "stamp": 212, "rms": 216, "nlabl": 220,
+           "labels": 224}
+
+
+def test_header_layout(mtr):
+    dt = mtr.mrc.HEADER_DTYPE
+    assert dt.itemsize == 1024
+    for name, off in OFFSETS.items():
+        assert dt.fields[name][1] == off, name
+
+
+def raw_mrc(path, data, mode, next_bytes=0):
+    nz, ny, nx = data.shape
+    hdr = bytearray(1024)
+    struct.pack_into("<4i", hdr, 0, nx, ny, nz, mode)
+    struct.pack_into("<i", hdr, 92, next_bytes)
+    with open(path, "wb") as f:
+        f.write(hdr)
+        f.write(b"\x5a" * next_bytes)
+        f.write(data.tobytes())
+
+
+@pytest.mark.parametrize("mode,dtype", [(0, np.uint8), (1, np.int16), (2, np.float32), (6, np.uint16)])
+@pytest.mark.parametrize("next_bytes", [0, 96])
+def test_read_modes(mtr, tmp_path, mode, dtype, next_bytes):
+    rng = np.random.default_rng(mode)
+    info = np.iinfo(dtype) if np.issubdtype(dtype, np.integer) else None
+    if info is not None:
+        d = rng.integers(info.min, info.max, (3, 5, 7), endpoint=True).astype(dtype)
+    else:
+        d = rng.normal(size=(3, 5, 7)).astype(dtype)
+    p = str(tmp_path / "a.mrc")
+    raw_mrc(p, d, mode, next_bytes)
+    h = mtr.mrc.read_header(p)
+    assert (int(h["nx"]), int(h["ny"]), int(h["nz"]), int(h["mode"])) == (7, 5, 3, mode)
+    assert np.array_equal(mtr.mrc.read(p), d.astype(np.float32))
+
+
+def test_bad_files_are_errors(mtr, tmp_path):
+    p = str(tmp_path / "t.mrc")
+    raw_mrc(p, np.zeros((2, 2, 2), np.float32), 2)
+    with open(p, "r+b") as f:
+        f.truncate(1024 + 20)
+    with pytest.raises(mtr.MtrError):
+        mtr.mrc.read(p)
+    raw_mrc(p, np.zeros((2, 2, 2), np.float32), 4)  # complex modes are not tilt data
+    with pytest.raises(mtr.MtrError):
+        mtr.mrc.read_header(p)
+    with pytest.raises(mtr.MtrError):
+        mtr.mrc.read_header(str(tmp_path / "missing.mrc"))
+
+
+def test_volume_round_trip_and_stats(mtr, tmp_path):
+    vol = mtr.volume_size(6, 5, 4, 10, -3, 2)
+    d = np.random.default_rng(0).normal(size=(4, 5, 6)).astype(np.float32)
+    p = str(tmp_path / "v.mrc")
+    mtr.mrc.write_volume(p, vol, d)
+    assert np.array_equal(mtr.mrc.read(p), d)
+    h = mtr.mrc.read_header(p)
+    assert (int(h["nxstart"]), int(h["nystart"]), int(h["nzstart"])) == (10, -3, 2)
+    assert h["amin"] == d.min() and h["amax"] == d.max()
+    assert abs(float(h["amean"]) - float(d.astype(np.float64).mean())) < 1e-6
+    assert bytes(h["cmap"]) == b"MAP " and int(h["mode"]) == 2
+
+
+def test_txbr_round_trip_and_union(mtr, tmp_path):
+    vol = mtr.volume_size(30, 20, 10, -5, 0, 3)
+    prj = mtr.projection_size(30, 20, 7)
+    xc, yc = mtr.synth.geometry(vol, prj, mtr.synth.GEOM_SLAB, 33.0, np.linspace(-60, 60, 7))
+    p = str(tmp_path / "g.txbr")
+    mtr.mrc.write_txbr(p, vol, xc, yc)
+    v2, x2, y2 = mtr.mrc.read_txbr(p)
+    assert np.array_equal(x2, xc) and np.array_equal(y2, yc)
+    assert (v2.Xstart, v2.Xend, v2.X, v2.Ystart, v2.Y, v2.Zstart, v2.Zend) == (-5, 25, 30, 0, 20, 3, 13)
+    # second file with a different box: the box becomes the union
+    q = str(tmp_path / "h.txbr")
+    mtr.mrc.write_txbr(q, mtr.volume_size(10, 40, 5, 0, -4, 0), xc, yc)
+    v3, _, _ = mtr.mrc.read_txbr(q, vol=v2)
+    assert (v3.Xstart, v3.Xend, v3.Ystart, v3.Yend, v3.Zstart, v3.Zend) == (-5, 25, -4, 36, 0, 13)
+
+
+def test_txbr_grammar_whitespace(mtr, tmp_path):
+    p = str(tmp_path / "w.txbr")
+    with open(p, "w") as f:
+        f.write("x->0:1:4\ny->0:1:3\nz->0:1:2\nl-1\n")
+        f.write("x-1 1.5\t0.25 0 1 \n")  # tab and trailing blank: still four numbers
+        f.write("y-1  2 0 1 0\n")
+    v, xc, yc = mtr.mrc.read_txbr(p)
+    assert xc.tolist() == [1.5, 0.25, 0.0, 1.0] and yc.tolist() == [2.0, 0.0, 1.0, 0.0]
+    with open(p, "a") as f:
+        f.write("x-2 1 2 3\n")  # x and y counts now differ
+    with pytest.raises(mtr.MtrError):
+        mtr.mrc.read_txbr(p)
+
+
+def test_cli_help_and_bad_input():
+    r = subprocess.run([CLI, "-h"], capture_output=True, text=True)
+    assert r.returncode == 0 and "-i K" in r.stdout
+    r = subprocess.run([CLI, "-i", "2,a.st", "-o", "x.mrc"], capture_output=True, text=True)
+    assert r.returncode == 2
+    r = subprocess.run([CLI, "-q"], capture_output=True, text=True)
+    assert r.returncode == 2
+
+
+def write_problem(mtr, pb, d):
+    sts, txs = [], []
+    for a in range(pb.K):
+        st, tx = str(d / f"s{a}.st"), str(d / f"s{a}.txbr")
+        mtr.mrc.write_stack(st, pb.ps[a])
+        mtr.mrc.write_txbr(tx, pb.vol, pb.xcs[a], pb.ycs[a])
+        sts.append(st)
+        txs.append(tx)
+    return sts, txs
+
+
+@pytest.mark.gpu
+def test_cli_reconstruction_matches_oracle(mtr, orc, tmp_path):
+    pb = mtr.synth.make_problem("cli", 28, 24, 10, 6, [0.0, 90.0], seed=21, amin=2, amax=6,
+                                theta=np.linspace(-60, 60, 9))
+    sts, txs = write_problem(mtr, pb, tmp_path)
+    out = str(tmp_path / "out.mrc")
+    print("[gpu] cli run", flush=True)
+    r = subprocess.run([CLI, "-i", ",".join([str(pb.K)] + sts + txs), "-o", out, "-n", "2", "-s", "0.5", "-m", "0.5",
+                        "--sirt-iters", "3", "-x", "8", "-y", "8", "-z", "8"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = mtr.mrc.read(out)
+    ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 2, 3, 0.5, 0.5)
+    assert orc.rel_l2(got, ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_reference_shaped_host_functions(mtr, orc, tmp_path):
+    pb = mtr.synth.make_problem("compat", 26, 22, 9, 6, [45.0], seed=22, amin=2, amax=6,
+                                theta=np.linspace(-60, 60, 9))
+    sts, txs = write_problem(mtr, pb, tmp_path)
+    print("[gpu] compat check", flush=True)
+    r = subprocess.run([CHECK, sts[0], txs[0], str(tmp_path), "0.5", "3", "0.5"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    bpt = orc.bpt(prj, vol, xc, yc, p)
+    got = {k: mtr.mrc.read(str(tmp_path / f"{k}.mrc")) for k in ("bpt", "bpt_slabs", "sirt", "z", "mann",
+                                                                    "mann_cpu_name")}
+    assert orc.rel_l2(got["bpt"], bpt) < 2e-6
+    assert np.array_equal(got["bpt_slabs"], got["bpt"])  # slabs stitch exactly
+    sirt = orc.sirt(prj, vol, xc, yc, p, got["bpt"], 3, 0.5)
+    assert orc.rel_l2(got["sirt"], sirt) < 1e-5
+    z = orc.form_z(got["bpt"], got["sirt"])
+    assert np.array_equal(got["z"], z)
+    assert np.array_equal(got["mann"], orc.mann(got["sirt"], got["bpt"], z, 0.5))
+    assert np.array_equal(got["mann_cpu_name"], got["mann"])
