@@ -83,7 +83,8 @@ int make_plan(const double *xc, const double *yc, FpPlan &pl)
             L = A2;
             M = A1;
         }
-        if (a[M] != 0.0) {
+        const double EmM = a[M] != 0.0 ? 1.0 / std::fabs(a[M]) : 0.0;
+        if (a[M] != 0.0 && std::floor(2.0 * EmM) + 1 <= 64) {
             pl.kind = 1;
             pl.W = W;
             pl.L = L;
@@ -95,6 +96,7 @@ int make_plan(const double *xc, const double *yc, FpPlan &pl)
             pl.mw = -a[W] / a[M];
             pl.ml = -a[L] / a[M];
             pl.Em = 1.0 / std::fabs(a[M]);
+            pl.ncand = (int)std::floor(2.0 * pl.Em) + 1;
             return MTR_OK;
         }
     }
@@ -1009,7 +1011,7 @@ static int op_bp_common(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_
     {
         Timed tm(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
         HIP_TRY(launch_bp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, bp_blocks(t.dev, ctx->bp_tz), ctx->bp_tz,
-                          bpt, lambda, nullptr, ctx->lds_cap, ctx->stream));
+                          bpt, lambda, t.r.p, ctx->lds_cap, ctx->stream));
     }
     HIP_TRY(hipMemcpyAsync(x, t.x.p, sizeof(float) * N, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

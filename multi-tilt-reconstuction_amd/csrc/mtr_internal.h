@@ -32,6 +32,8 @@ struct FpPlan {
     int D, P, Q;   // general: driving axis and in-plane axes (0=X,1=Y,2=Z)
     int W, L, M;   // separable: plane axis, line (drive) axis, cross axis
     int wsign;     // separable: bW (+1/-1)
+    int ncand;     // separable: lattice points per line, floor(2 Em) + 1
+    int pad;
     double woff;   // separable: W = (k + 1 - yc0) * wsign
     // general: crossing point (p*, q*) = p0 + i*pi + k*pk + d*pd
     double p0, pi, pk, pd;

@@ -161,9 +161,14 @@ __device__ FpAcc fp_ray_general(const AgentDev &ag, const ViewCoef &vc, const Fp
     return acc;
 }
 
-template <class T, bool UNIT_X>
-__device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const FpPlan &pl, int i,
-                                  int k)
+// Row-separable views: detector row k is plane W(k) of the volume with
+// weight (1 - dv) = 1, and inside the plane u = alpha + aL*l + aM*m.  Each
+// line l of the drive axis meets the ray at m*(l); the support
+// |u - i| < 1 is the open interval m*(l) +- 1/|aM|, which holds at most
+// ncand lattice points, so every lane runs the same fixed-length loop with
+// tent weights max(0, 1 - |t|), t = aM (m - m*), in FP32.
+template <class T, bool NEED_C, bool UNIT_X>
+__device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const FpPlan &pl, int i, int k)
 {
     FpAcc acc{0.f, 0.f};
     const T *x = reinterpret_cast<const T *>(ag.x);
@@ -186,21 +191,25 @@ __device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const 
     const int64_t sW = sel3<int64_t>(1, sy, sz, pl.W), sL = sel3<int64_t>(1, sy, sz, pl.L),
                   sM = sel3<int64_t>(1, sy, sz, pl.M);
     const int64_t base = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz + w * sW;
-    const double *cu = vc.c;
+    const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
+    const float aMf = (float)aM;
+    const unsigned mspan = (unsigned)(Mhi - Mlo);
+    const int nc = pl.ncand;
     for (int l = l0; l <= l1; l++) {
         const double mc = ms + l * pl.ml;
-        const int m0 = max(Mlo, (int)ceil(mc - pl.Em - 1e-9));
-        const int m1 = min(Mhi, (int)floor(mc + pl.Em + 1e-9));
-        for (int m = m0; m <= m1; m++) {
-            const int X = pl.W == 0 ? w : (pl.L == 0 ? l : m);
-            const int Y = pl.W == 1 ? w : (pl.L == 1 ? l : m);
-            const int Z = pl.W == 2 ? w : (pl.L == 2 ? l : m);
-            const double u = coord(cu, X, Y, Z);
-            const double tu = u - (double)i;
-            if (fabs(tu) < 1.0) {
-                const double wu = tu >= 0.0 ? 1.0 - tu : u - (double)(i - 1);
-                // (1 - dv) == 1 exactly: v is integral on this row.
-                fp_take<T, UNIT_X>(acc, x, base + l * sL + m * sM, wu * 1.0);
+        const double mf = floor(mc - pl.Em) + 1.0;  // first lattice point inside the support
+        int m = (int)mf;
+        float t = (float)(aM * (mf - mc));           // u - i at m
+        const int64_t lb = base + l * sL;
+        for (int j = 0; j < nc; j++, m++, t += aMf) {
+            const bool ok = (unsigned)(m - Mlo) <= mspan;
+            const float wgt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+            if (NEED_C) acc.c += wgt;
+            if (UNIT_X) {
+                acc.s += wgt;
+            } else {
+                const int mm = ok ? m : Mlo;
+                acc.s = fmaf(wgt, ldf<T>(x, lb + mm * sM), acc.s);
             }
         }
     }
@@ -230,7 +239,7 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
         const ViewCoef vc = ag.coef[a];
         const FpPlan pl = ag.plan[a];
         if (pl.kind == 1)
-            acc = fp_ray_separable<T, MODE == 2>(ag, vc, pl, i, k);
+            acc = fp_ray_separable<T, MODE != 0, MODE == 2>(ag, vc, pl, i, k);
         else
             acc = fp_ray_general<T, MODE == 2>(ag, vc, pl, i, k);
     }
@@ -289,7 +298,8 @@ __global__ __launch_bounds__(256) void k_bp(const AgentDev *__restrict__ ags, fl
     const int ex = min(64, ag.Xe - X0) - 1, ey = min(4, ag.Ye - Y0) - 1, ez = min(TZ, ag.Ze - Z0) - 1;
     const int nu = ag.nu, nv = ag.nv;
     const int64_t plane = (int64_t)nu * nv;
-    const T *rsrc = src_override ? src_override : reinterpret_cast<const T *>(ag.r);
+    // BPT back-projects the projections, SIRT the residual.
+    const T *rsrc = src_override ? src_override : reinterpret_cast<const T *>(BPT ? ag.p : ag.r);
 
     float S[TZ], C[TZ];
 #pragma unroll
@@ -326,6 +336,41 @@ __global__ __launch_bounds__(256) void k_bp(const AgentDev *__restrict__ ags, fl
             __syncthreads();
         }
         double u = coord(cu, X, Y, Z0), v = coord(cv, X, Y, Z0);
+        if (ag.plan[a].kind == 1) {
+            // Row-separable view: v is integral, dv == 0, the k0 + 1 taps
+            // carry weight 0 and only the two u-taps of row k0 remain.
+#pragma unroll
+            for (int t = 0; t < TZ; t++) {
+                const double fu = floor(u);
+                const int i0 = (int)fu, k0 = (int)v;
+                const float du = (float)(u - fu);
+                float r00, r01;
+                if (staged) {
+                    const int li = min(max(i0 - fi0, 0), W - 2), lk = min(max(k0 - fk0, 0), H - 2);
+                    const float *q = lds + lk * W + li;
+                    r00 = q[0];
+                    r01 = q[1];
+                } else {
+                    const int ia = min(max(i0, 0), nu - 1), ib = min(max(i0 + 1, 0), nu - 1);
+                    const int ka = min(max(k0, 0), nv - 1);
+                    r00 = ldf<T>(rv, (int64_t)ka * nu + ia);
+                    r01 = ldf<T>(rv, (int64_t)ka * nu + ib);
+                }
+                if (interior) {
+                    S[t] += fmaf(du, r01 - r00, r00);
+                    C[t] += 1.0f;
+                } else {
+                    const bool kv = (unsigned)k0 < (unsigned)nv;
+                    const float wu0 = kv && (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
+                    const float wu1 = kv && (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
+                    S[t] += wu0 * r00 + wu1 * r01;
+                    C[t] += wu0 + wu1;
+                }
+                u += cu[3];
+                v += cv[3];
+            }
+            continue;
+        }
 #pragma unroll
         for (int t = 0; t < TZ; t++) {
             const double fu = floor(u), fv = floor(v);
