@@ -127,6 +127,33 @@ int make_plan(const double *xc, const double *yc, FpPlan &pl)
     return MTR_OK;
 }
 
+// Back-projection mode of an agent: the run-along-W kernel applies when
+// every view is row-separable along the same axis W, u does not depend on
+// W, and the tile footprint fits the fixed LDS pitch.
+struct BpMode {
+    bool sep = false;
+    int W = 0, P1 = 0, P2 = 0;
+};
+
+BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc)
+{
+    BpMode m;
+    if (plan.empty()) return m;
+    const int W = plan[0].W;
+    for (size_t v = 0; v < plan.size(); v++)
+        if (plan[v].kind != 1 || plan[v].W != W || xc[4 * v + 1 + W] != 0.0) return m;
+    const int P1 = W == 0 ? 1 : 0, P2 = 3 - W - P1;
+    for (size_t v = 0; v < plan.size(); v++) {
+        const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * 3 + 5;
+        if (w > bp_sep_pitch()) return m;
+    }
+    m.sep = true;
+    m.W = W;
+    m.P1 = P1;
+    m.P2 = P2;
+    return m;
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf()
@@ -142,6 +169,7 @@ struct Agent {
     mtr_projection_size prj{};
     std::vector<double> xc, yc;
     bool lanes_k = false;  // forward projection with lanes along detector rows
+    BpMode bpm;            // back-projection kernel choice
     ViewCoef *d_coef = nullptr;
     FpPlan *d_plan = nullptr;
     void *d_p = nullptr;
@@ -329,10 +357,24 @@ int fp_group(mtr_ctx *ctx, const std::vector<int> &ids)
 
 int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
 {
-    const int mb = bp_blocks(make_dev(ctx, ctx->agents[first]), ctx->bp_tz);
-    Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
-    HIP_TRY(launch_bp(ctx->precision, ctx->d_ags + first, count, mb, ctx->bp_tz, bpt, lambda, nullptr,
-                      ctx->lds_cap, ctx->stream));
+    // launch each maximal run of agents sharing a back-projection mode
+    int s = first;
+    const int end = first + count;
+    while (s < end) {
+        const BpMode &m = ctx->agents[s].bpm;
+        int e = s + 1;
+        while (e < end && ctx->agents[e].bpm.sep == m.sep && ctx->agents[e].bpm.W == m.W) e++;
+        const AgentDev d = make_dev(ctx, ctx->agents[s]);
+        Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
+        if (m.sep) {
+            HIP_TRY(launch_bp_sep(ctx->precision, ctx->d_ags + s, e - s, bp_sep_blocks(d, m.W, m.P1, m.P2), bpt, lambda,
+                                  m.W, m.P1, m.P2, ctx->stream));
+        } else {
+            HIP_TRY(launch_bp(ctx->precision, ctx->d_ags + s, e - s, bp_blocks(d, ctx->bp_tz), ctx->bp_tz, bpt, lambda,
+                              nullptr, ctx->lds_cap, ctx->stream));
+        }
+        s = e;
+    }
     return MTR_OK;
 }
 
@@ -434,6 +476,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     bool all_wx = true;
     for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     ag.lanes_k = all_wx;
+    ag.bpm = bp_mode(plan, xc);
     const size_t es = elem_size(ctx->precision);
     const int64_t P = ag.P();
 #define ALLOC(ptr, bytes)                                                                 \
@@ -500,6 +543,7 @@ struct TmpAgent {
     DevBuf coef, plan, x, s, c, r, tab;
     AgentDev dev{};
     bool lanes_k = false;
+    BpMode bpm;
     int nfp = 0;
 };
 
@@ -522,6 +566,7 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
         all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     }
     t.lanes_k = all_wx;
+    t.bpm = bp_mode(plan, xc);
     const int64_t P = (int64_t)prj->X * prj->Y * A;
     const int64_t N = (int64_t)vol->X * vol->Y * vol->Z;
     HIP_TRY(hipMalloc(&t.coef.p, sizeof(ViewCoef) * A));
@@ -548,7 +593,7 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     d.Z = vol->Z;
     d.coef = (const ViewCoef *)t.coef.p;
     d.plan = (const FpPlan *)t.plan.p;
-    d.p = nullptr;
+    d.p = t.r.p;  // the stateless BP/BPT read their input from r
     d.c = (const float *)t.c.p;
     d.r = t.r.p;
     d.x = t.x.p;
@@ -1010,8 +1055,13 @@ static int op_bp_common(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_
     if (!bpt) HIP_TRY(hipMemcpyAsync(t.x.p, x, sizeof(float) * N, hipMemcpyHostToDevice, ctx->stream));
     {
         Timed tm(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
-        HIP_TRY(launch_bp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, bp_blocks(t.dev, ctx->bp_tz), ctx->bp_tz,
-                          bpt, lambda, t.r.p, ctx->lds_cap, ctx->stream));
+        if (t.bpm.sep)
+            HIP_TRY(launch_bp_sep(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1,
+                                  bp_sep_blocks(t.dev, t.bpm.W, t.bpm.P1, t.bpm.P2), bpt, lambda, t.bpm.W, t.bpm.P1,
+                                  t.bpm.P2, ctx->stream));
+        else
+            HIP_TRY(launch_bp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, bp_blocks(t.dev, ctx->bp_tz), ctx->bp_tz,
+                              bpt, lambda, t.r.p, ctx->lds_cap, ctx->stream));
     }
     HIP_TRY(hipMemcpyAsync(x, t.x.p, sizeof(float) * N, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

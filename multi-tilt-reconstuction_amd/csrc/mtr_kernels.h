@@ -8,6 +8,11 @@ namespace mtr {
 int fp_blocks(const AgentDev &ag, bool lanes_k);
 int bp_blocks(const AgentDev &ag, int tz);
 int consensus_blocks(int64_t N);
+// Row-separable back-projection (runs along the plane axis W).
+int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2);
+int bp_sep_pitch();
+hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
+                         int P1, int P2, hipStream_t st);
 
 // Forward projection of nag agents (device table d_ags), mode 0 = fused
 // residual, 1 = raw s/c into s_out/c_out, 2 = normaliser into c_out.

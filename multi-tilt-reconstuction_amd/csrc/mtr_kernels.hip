@@ -161,6 +161,78 @@ __device__ FpAcc fp_ray_general(const AgentDev &ag, const ViewCoef &vc, const Fp
     return acc;
 }
 
+// One line of a row-separable ray: the NC lattice points of the support
+// starting at the first point inside it, with FP32 tent weights.
+template <class T, int NC>
+__device__ __forceinline__ void sep_line_load(const T *x, int64_t lb, int64_t sM, double mc, const FpPlan &pl,
+                                              double aM, float aMf, int Mlo, unsigned mspan, float (&wv)[NC],
+                                              float (&xv)[NC], bool unit_x)
+{
+    const double mf = floor(mc - pl.Em) + 1.0;
+    int m = (int)mf;
+    float t = (float)(aM * (mf - mc));
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
+        const bool ok = (unsigned)(m - Mlo) <= mspan;
+        wv[j] = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+        if (!unit_x) xv[j] = ldf<T>(x, lb + (ok ? m : Mlo) * sM);
+        m++;
+        t += aMf;
+    }
+}
+
+template <class T, bool NEED_C, bool UNIT_X, int NC>
+__device__ __forceinline__ void sep_lines(FpAcc &acc, const T *x, int64_t base, int64_t sL, int64_t sM, double ms,
+                                          const FpPlan &pl, double aM, float aMf, int Mlo, unsigned mspan, int l0,
+                                          int l1)
+{
+    constexpr int U = 4;  // lines in flight per lane
+    int l = l0;
+    for (; l + U - 1 <= l1; l += U) {
+        float wv[U][NC], xv[U][NC];
+#pragma unroll
+        for (int q = 0; q < U; q++)
+            sep_line_load<T, NC>(x, base + (l + q) * sL, sM, ms + (l + q) * pl.ml, pl, aM, aMf, Mlo, mspan, wv[q],
+                                 xv[q], UNIT_X);
+#pragma unroll
+        for (int q = 0; q < U; q++)
+#pragma unroll
+            for (int j = 0; j < NC; j++) {
+                if (NEED_C) acc.c += wv[q][j];
+                acc.s = UNIT_X ? acc.s + wv[q][j] : fmaf(wv[q][j], xv[q][j], acc.s);
+            }
+    }
+    for (; l <= l1; l++) {
+        float wv[NC], xv[NC];
+        sep_line_load<T, NC>(x, base + l * sL, sM, ms + l * pl.ml, pl, aM, aMf, Mlo, mspan, wv, xv, UNIT_X);
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            if (NEED_C) acc.c += wv[j];
+            acc.s = UNIT_X ? acc.s + wv[j] : fmaf(wv[j], xv[j], acc.s);
+        }
+    }
+}
+
+template <class T, bool NEED_C, bool UNIT_X>
+__device__ void sep_lines_any(FpAcc &acc, const T *x, int64_t base, int64_t sL, int64_t sM, double ms,
+                              const FpPlan &pl, double aM, float aMf, int Mlo, unsigned mspan, int l0, int l1)
+{
+    for (int l = l0; l <= l1; l++) {
+        const double mc = ms + l * pl.ml;
+        const double mf = floor(mc - pl.Em) + 1.0;
+        int m = (int)mf;
+        float t = (float)(aM * (mf - mc));
+        const int64_t lb = base + l * sL;
+        for (int j = 0; j < pl.ncand; j++, m++, t += aMf) {
+            const bool ok = (unsigned)(m - Mlo) <= mspan;
+            const float wgt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+            if (NEED_C) acc.c += wgt;
+            if (UNIT_X) acc.s += wgt;
+            else acc.s = fmaf(wgt, ldf<T>(x, lb + (ok ? m : Mlo) * sM), acc.s);
+        }
+    }
+}
+
 // Row-separable views: detector row k is plane W(k) of the volume with
 // weight (1 - dv) = 1, and inside the plane u = alpha + aL*l + aM*m.  Each
 // line l of the drive axis meets the ray at m*(l); the support
@@ -194,24 +266,11 @@ __device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const 
     const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
     const float aMf = (float)aM;
     const unsigned mspan = (unsigned)(Mhi - Mlo);
-    const int nc = pl.ncand;
-    for (int l = l0; l <= l1; l++) {
-        const double mc = ms + l * pl.ml;
-        const double mf = floor(mc - pl.Em) + 1.0;  // first lattice point inside the support
-        int m = (int)mf;
-        float t = (float)(aM * (mf - mc));           // u - i at m
-        const int64_t lb = base + l * sL;
-        for (int j = 0; j < nc; j++, m++, t += aMf) {
-            const bool ok = (unsigned)(m - Mlo) <= mspan;
-            const float wgt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
-            if (NEED_C) acc.c += wgt;
-            if (UNIT_X) {
-                acc.s += wgt;
-            } else {
-                const int mm = ok ? m : Mlo;
-                acc.s = fmaf(wgt, ldf<T>(x, lb + mm * sM), acc.s);
-            }
-        }
+    switch (pl.ncand) {
+        case 2: sep_lines<T, NEED_C, UNIT_X, 2>(acc, x, base, sL, sM, ms, pl, aM, aMf, Mlo, mspan, l0, l1); break;
+        case 3: sep_lines<T, NEED_C, UNIT_X, 3>(acc, x, base, sL, sM, ms, pl, aM, aMf, Mlo, mspan, l0, l1); break;
+        case 4: sep_lines<T, NEED_C, UNIT_X, 4>(acc, x, base, sL, sM, ms, pl, aM, aMf, Mlo, mspan, l0, l1); break;
+        default: sep_lines_any<T, NEED_C, UNIT_X>(acc, x, base, sL, sM, ms, pl, aM, aMf, Mlo, mspan, l0, l1); break;
     }
     return acc;
 }
@@ -421,6 +480,162 @@ __global__ __launch_bounds__(256) void k_bp(const AgentDev *__restrict__ ags, fl
             stf<T>(x, j, C[t] != 0.0f ? S[t] / C[t] : 0.0f);
         } else if (C[t] != 0.0f) {
             stf<T>(x, j, fmaf(S[t] / C[t], lambda, ldf<T>(x, j)));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Back-projection of agents whose views are all row-separable with u
+// independent of the plane axis W (v = v0 + bW*W, u = u(P1, P2)): the slab
+// and axis-Z geometries.  Each thread owns a run of RW voxels along W at a
+// fixed (P1, P2); per view it forms u, floor(u) and du once, and the run's
+// voxels read detector rows k(W0 + t) of one column pair.  The block's
+// detector footprint (RW rows x at most PITCH columns, zero outside the
+// detector) is staged in LDS with a fixed pitch, double buffered: the next
+// view's rows are fetched into registers while the current view is
+// accumulated.  Lanes run along P1 (X, or Y when W = X), waves along P2.
+struct BpSepArgs {
+    int W, P1, P2;  // axes (0 = X, 1 = Y, 2 = Z)
+};
+
+template <int RW, int PITCH>
+struct BpSepStage {
+    static constexpr int kElems = RW * PITCH;
+    static constexpr int kPer = (kElems + 255) / 256;
+};
+
+template <class T, int RW, int PITCH>
+__device__ __forceinline__ void bp_sep_footprint(const AgentDev &ag, const ViewCoef &vc, const BpSepArgs &sa,
+                                                 int P10, int P20, int W0, int e1, int e2, int &fi0, int &width,
+                                                 int &k0row, int &kstep)
+{
+    const double *cu = vc.c, *cv = vc.c + 4;
+    const int X = sel3(P10, P20, W0, sa.P1 == 0 ? 0 : (sa.P2 == 0 ? 1 : 2));
+    const int Y = sel3(P10, P20, W0, sa.P1 == 1 ? 0 : (sa.P2 == 1 ? 1 : 2));
+    const int Z = sel3(P10, P20, W0, sa.P1 == 2 ? 0 : (sa.P2 == 2 ? 1 : 2));
+    const double u0 = coord(cu, X, Y, Z);
+    const double a1 = sel3(cu[1], cu[2], cu[3], sa.P1), a2 = sel3(cu[1], cu[2], cu[3], sa.P2);
+    const double ulo = u0 + fmin(0.0, a1 * e1) + fmin(0.0, a2 * e2);
+    const double uhi = u0 + fmax(0.0, a1 * e1) + fmax(0.0, a2 * e2);
+    fi0 = max((int)floor(ulo) - 1, -1);
+    const int fi1 = min((int)floor(uhi) + 2, ag.nu);
+    width = fi1 - fi0 + 1;
+    k0row = (int)coord(cv, X, Y, Z);  // v is integral; row of W0
+    kstep = (int)sel3(cv[1], cv[2], cv[3], sa.W);
+}
+
+template <class T, int RW, int PITCH, bool BPT>
+__global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda, BpSepArgs sa)
+{
+    __shared__ float lds[2][RW * PITCH];
+    const AgentDev &ag = ags[blockIdx.y];
+    const int n1 = sel3(ag.X, ag.Y, ag.Z, sa.P1), n2 = sel3(ag.X, ag.Y, ag.Z, sa.P2), nw = sel3(ag.X, ag.Y, ag.Z, sa.W);
+    const int s1 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P1), s2 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P2), sw = sel3(ag.Xs, ag.Ys, ag.Zs, sa.W);
+    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + 3) / 4, ntw = (nw + RW - 1) / RW;
+    const int nb = nt1 * nt2 * ntw;
+    // XCD-aware order: consecutive tiles (which share detector rows) run on
+    // the same XCD's L2.
+    const int per = (nb + 7) / 8;
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
+    const int b1 = b % nt1, b2 = (b / nt1) % nt2, bw = b / (nt1 * nt2);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * 4, W0 = sw + bw * RW;
+    const int p1 = P10 + lane, p2 = P20 + wid;
+    const int e1 = min(64, s1 + n1 - P10) - 1, e2 = min(4, s2 + n2 - P20) - 1, ew = min(RW, sw + nw - W0) - 1;
+    const bool active = p1 < s1 + n1 && p2 < s2 + n2;
+    const int nu = ag.nu, nv = ag.nv;
+    const int64_t plane = (int64_t)nu * nv;
+    const T *src = reinterpret_cast<const T *>(BPT ? ag.p : ag.r);
+    const int X = sa.P1 == 0 ? p1 : (sa.P2 == 0 ? p2 : W0);
+    const int Y = sa.P1 == 1 ? p1 : (sa.P2 == 1 ? p2 : W0);
+    const int Z = sa.P1 == 2 ? p1 : (sa.P2 == 2 ? p2 : W0);
+    using St = BpSepStage<RW, PITCH>;
+
+    float S[RW], C[RW];
+#pragma unroll
+    for (int t = 0; t < RW; t++) S[t] = C[t] = 0.f;
+    float Cint = 0.f;
+    float pre[St::kPer];
+
+    // fetch the footprint of view a into registers
+    auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
+        const ViewCoef vc = ag.coef[a];
+        bp_sep_footprint<T, RW, PITCH>(ag, vc, sa, P10, P20, W0, e1, e2, fi0, width, k0row, kstep);
+        const T *rv = src + plane * a;
+#pragma unroll
+        for (int q = 0; q < St::kPer; q++) {
+            const int e = threadIdx.x + 256 * q;
+            const int row = e / PITCH, col = e - row * PITCH;
+            const int gk = k0row + kstep * row, gi = fi0 + col;
+            float val = 0.f;
+            if (e < St::kElems && col < width && row <= ew && (unsigned)gk < (unsigned)nv && (unsigned)gi < (unsigned)nu)
+                val = ldf<T>(rv, (int64_t)gk * nu + gi);
+            pre[q] = val;
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < St::kPer; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < St::kElems) lds[buf][e] = pre[q];
+        }
+    };
+
+    int fi0, width, k0row, kstep;
+    fetch(0, fi0, width, k0row, kstep);
+    commit(0);
+    __syncthreads();
+    for (int a = 0; a < ag.A; a++) {
+        const int cfi0 = fi0, cwidth = width, ck0 = k0row, cks = kstep;
+        if (a + 1 < ag.A) fetch(a + 1, fi0, width, k0row, kstep);
+        const ViewCoef vc = ag.coef[a];
+        const double u = coord(vc.c, X, Y, Z);
+        const double fu = floor(u);
+        const int i0 = (int)fu;
+        const float du = (float)(u - fu);
+        const int li = min(max(i0 - cfi0, 0), min(cwidth, PITCH) - 2);
+        const float *q = &lds[a & 1][li];
+        const int klast = ck0 + cks * ew;
+        const bool rows_in = (unsigned)ck0 < (unsigned)nv && (unsigned)klast < (unsigned)nv;
+        const bool interior = rows_in && cfi0 >= 0 && cfi0 + cwidth - 1 <= nu - 1;
+        if (cwidth < 2) {
+            // the view misses this tile: no tap is on the detector
+        } else if (interior) {
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                const float r0 = q[t * PITCH], r1 = q[t * PITCH + 1];
+                S[t] += fmaf(du, r1 - r0, r0);
+            }
+            Cint += 1.0f;
+        } else {
+            const float wu0 = (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
+            const float wu1 = (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                const float r0 = q[t * PITCH], r1 = q[t * PITCH + 1];
+                const bool rv = (unsigned)(ck0 + cks * t) < (unsigned)nv;
+                S[t] += wu0 * r0 + wu1 * r1;
+                C[t] += rv ? wu0 + wu1 : 0.f;
+            }
+        }
+        if (a + 1 < ag.A) commit((a + 1) & 1);
+        __syncthreads();
+    }
+    if (!active) return;
+    T *x = reinterpret_cast<T *>(ag.x);
+    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
+    const int64_t sW = sel3<int64_t>(1, sy, sz, sa.W);
+    const int64_t j0 = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * sy + (int64_t)(Z - ag.Zs) * sz;
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        if (t > ew) break;
+        const int64_t j = j0 + t * sW;
+        const float Ct = Cint + C[t];
+        if (BPT) {
+            stf<T>(x, j, Ct != 0.0f ? S[t] / Ct : 0.0f);
+        } else if (Ct != 0.0f) {
+            stf<T>(x, j, fmaf(S[t] / Ct, lambda, ldf<T>(x, j)));
         }
     }
 }
@@ -778,6 +993,39 @@ hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, i
     if (prec == MTR_PREC_FP16)
         return launch_bp_t<__half>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
     return launch_bp_t<float>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
+}
+
+constexpr int kSepRW = 16, kSepPitch = 80;
+
+int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2)
+{
+    auto ext = [&](int ax) { return ax == 0 ? ag.X : (ax == 1 ? ag.Y : ag.Z); };
+    const int nb = ((ext(P1) + 63) / 64) * ((ext(P2) + 3) / 4) * ((ext(W) + kSepRW - 1) / kSepRW);
+    return ((nb + 7) / 8) * 8;
+}
+
+int bp_sep_pitch() { return kSepPitch; }
+
+template <class T>
+static hipError_t launch_bp_sep_t(const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
+                                  int P1, int P2, hipStream_t st)
+{
+    BpSepArgs sa{W, P1, P2};
+    dim3 grid(max_blocks, nag), block(256);
+    if (bpt)
+        hipLaunchKernelGGL((k_bp_sep<T, kSepRW, kSepPitch, true>), grid, block, 0, st, d_ags, lambda, sa);
+    else
+        hipLaunchKernelGGL((k_bp_sep<T, kSepRW, kSepPitch, false>), grid, block, 0, st, d_ags, lambda, sa);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
+                         int P1, int P2, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16)
+        return launch_bp_sep_t<__hip_bfloat16>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
+    if (prec == MTR_PREC_FP16) return launch_bp_sep_t<__half>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
+    return launch_bp_sep_t<float>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
 }
 
 int consensus_blocks(int64_t N) { return ew_grid(N) > 8192 ? 8192 : ew_grid(N); }
