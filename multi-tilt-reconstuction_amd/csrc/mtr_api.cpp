@@ -154,6 +154,20 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc)
     return m;
 }
 
+// Forward-projection streaming mode: every view row-separable along the
+// same axis W, W contiguous in memory (W = X), u independent of W; returns
+// the candidate count to instantiate (0 = not applicable).
+int fp_stream_nc(const std::vector<FpPlan> &plan, const double *xc)
+{
+    if (plan.empty() || plan[0].kind != 1 || plan[0].W != 0) return 0;
+    int nc = 0;
+    for (size_t v = 0; v < plan.size(); v++) {
+        if (plan[v].kind != 1 || plan[v].W != 0 || xc[4 * v + 1] != 0.0) return 0;
+        nc = std::max(nc, plan[v].ncand);
+    }
+    return (nc >= 2 && nc <= 4) ? nc : 0;
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf()
@@ -170,6 +184,8 @@ struct Agent {
     std::vector<double> xc, yc;
     bool lanes_k = false;  // forward projection with lanes along detector rows
     BpMode bpm;            // back-projection kernel choice
+    int fp_nc = 0;         // > 0: streaming forward projection with this candidate count
+    int npartial = 0;      // per-block partials written by the last FP launch
     ViewCoef *d_coef = nullptr;
     FpPlan *d_plan = nullptr;
     void *d_p = nullptr;
@@ -334,22 +350,30 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
     return MTR_OK;
 }
 
-// One fused FP+residual launch over a set of agents sharing a lane mode.
+// One fused FP+residual launch per maximal run of consecutive agents that
+// share a forward-projection kernel.
 int fp_group(mtr_ctx *ctx, const std::vector<int> &ids)
 {
-    // Agents are contiguous in the device table only by index; launch each
-    // maximal run of consecutive ids with the same lane mode together.
     size_t s = 0;
     while (s < ids.size()) {
+        const Agent &a0 = ctx->agents[ids[s]];
         size_t e = s + 1;
-        while (e < ids.size() && ids[e] == ids[e - 1] + 1 &&
-               ctx->agents[ids[e]].lanes_k == ctx->agents[ids[s]].lanes_k)
+        while (e < ids.size() && ids[e] == ids[e - 1] + 1 && ctx->agents[ids[e]].lanes_k == a0.lanes_k &&
+               ctx->agents[ids[e]].fp_nc == a0.fp_nc)
             e++;
         int mb = 0;
-        for (size_t q = s; q < e; q++) mb = std::max(mb, ctx->agents[ids[q]].nfp);
+        for (size_t q = s; q < e; q++) {
+            Agent &ag = ctx->agents[ids[q]];
+            const AgentDev d = make_dev(ctx, ag);
+            ag.npartial = ag.fp_nc ? fp_stream_blocks(d) : ag.nfp;
+            mb = std::max(mb, ag.npartial);
+        }
         Timed t(ctx, MTR_K_FP);
-        HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, 0,
-                          ctx->agents[ids[s]].lanes_k, nullptr, nullptr, ctx->stream));
+        if (a0.fp_nc)
+            HIP_TRY(launch_fp_stream(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, a0.fp_nc, ctx->stream));
+        else
+            HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, 0, a0.lanes_k, nullptr, nullptr,
+                              ctx->stream));
         s = e;
     }
     return MTR_OK;
@@ -477,6 +501,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     ag.lanes_k = all_wx;
     ag.bpm = bp_mode(plan, xc);
+    ag.fp_nc = fp_stream_nc(plan, xc);
     const size_t es = elem_size(ctx->precision);
     const int64_t P = ag.P();
 #define ALLOC(ptr, bytes)                                                                 \
@@ -493,7 +518,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ALLOC(ag.d_x, es * ctx->N);
     ALLOC(ag.d_w, es * ctx->N);
     ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
-    ALLOC(ag.d_partial, sizeof(double) * std::max(ag.nfp, 1));
+    ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag))), 1));
     ALLOC(ag.d_scalar, sizeof(double));
     if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
@@ -932,7 +957,8 @@ int mtr_residual_norm2(mtr_ctx *ctx, int agent, double *out)
     if (!out) return fail(MTR_ERR_ARG, "null out");
     MTR_TRY(set_device(ctx));
     Agent &ag = ctx->agents[agent];
-    HIP_TRY(launch_reduce(ag.d_partial, ag.nfp, ag.d_scalar, ctx->stream));
+    if (ag.npartial <= 0) return fail(MTR_ERR_STATE, "no SIRT iteration has run on agent %d", agent);
+    HIP_TRY(launch_reduce(ag.d_partial, ag.npartial, ag.d_scalar, ctx->stream));
     HIP_TRY(hipMemcpyAsync(out, ag.d_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MTR_OK;

@@ -328,6 +328,117 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
 }
 
 // ------------------------------------------------------------------------
+// Forward projection of row-separable agents whose plane axis W is the
+// fastest-varying axis of the volume and whose u does not depend on W.
+// Block: 4 waves, each one detector column i of one view; the 64 lanes are
+// 64 consecutive detector rows k, i.e. 64 adjacent planes W(k).  All lanes
+// share the ray path, so the support is generated once per wave: lane q
+// builds candidate q of the current chunk, a ballot keeps the taps with
+// nonzero weight, and their (offset, weight) pairs are broadcast with
+// readlane; each tap is then one coalesced 256-byte load and one FMA for
+// 64 pixels.  Taps are consumed in (line, candidate) order, the order of
+// the per-lane path, so both give identical sums.
+template <class T, int NC>
+__global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ ags)
+{
+    const AgentDev &ag = ags[blockIdx.y];
+    const int nti = (ag.nu + 3) / 4, ntk = (ag.nv + 63) / 64;
+    int b = blockIdx.x;
+    if (b >= nti * ntk * ag.A) return;
+    const int bi = b % nti;
+    b /= nti;
+    const int bk = b % ntk;
+    const int a = b / ntk;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int i = bi * 4 + wid;
+    const int k = bk * 64 + lane;
+    const FpPlan pl = ag.plan[a];
+    const ViewCoef vc = ag.coef[a];
+    const T *x = reinterpret_cast<const T *>(ag.x);
+    const int Wlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.W), Whi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.W) - 1;
+    const int w = k * pl.wsign + (int)pl.woff;
+    const bool lane_ok = i < ag.nu && k < ag.nv && w >= Wlo && w <= Whi;
+    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
+    const int64_t sL = sel3<int64_t>(1, sy, sz, pl.L), sM = sel3<int64_t>(1, sy, sz, pl.M),
+                  sW = sel3<int64_t>(1, sy, sz, pl.W);
+    const int64_t base0 = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz;
+    const T *xl = x + base0 + (lane_ok ? w : Wlo) * sW;  // this lane's plane
+    float acc = 0.f;
+    if (i < ag.nu) {
+        const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
+        const int Mlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.M), Mhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.M) - 1;
+        const double ms = pl.m0 + i * pl.mi;  // aW == 0: no plane term
+        double llo = Llo, lhi = Lhi;
+        bool any = true;
+        if (pl.ml != 0.0) {
+            double t0 = (Mlo - pl.Em - 1.0 - ms) / pl.ml, t1 = (Mhi + pl.Em + 1.0 - ms) / pl.ml;
+            llo = fmax(llo, fmin(t0, t1));
+            lhi = fmin(lhi, fmax(t0, t1));
+        } else if (ms < Mlo - pl.Em - 1.0 || ms > Mhi + pl.Em + 1.0) {
+            any = false;
+        }
+        if (any && llo <= lhi) {
+            const int l0 = max(Llo, (int)floor(llo)), l1 = min(Lhi, (int)ceil(lhi));
+            const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
+            const unsigned mspan = (unsigned)(Mhi - Mlo);
+            const int n = (l1 - l0 + 1) * NC;
+            for (int c0 = 0; c0 < n; c0 += 64) {
+                // candidate c of the ray: line l0 + c / NC, point c % NC
+                const int c = c0 + lane;
+                const int l = l0 + c / NC, j = c - (c / NC) * NC;
+                const double mc = ms + l * pl.ml;
+                const double mf = floor(mc - pl.Em) + 1.0;
+                const int m = (int)mf + j;
+                // same FP32 steps as the per-lane path: t0, t0 + aM, t0 + aM + aM, ...
+                const float aMf = (float)aM;
+                float t = (float)(aM * (mf - mc));
+#pragma unroll
+                for (int q = 1; q < NC; q++)
+                    if (q <= j) t += aMf;
+                const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
+                const float wt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+                const int off = ok ? (int)(l * sL + m * sM) : 0;
+                uint64_t mask = __ballot(wt > 0.0f);
+                while (mask) {
+                    int o[8];
+                    float ww[8], xv[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        if (mask) {
+                            const int e = __builtin_ctzll(mask);
+                            mask &= mask - 1;
+                            o[q] = __builtin_amdgcn_readlane(off, e);
+                            ww[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wt), e));
+                        } else {
+                            o[q] = 0;
+                            ww[q] = 0.0f;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) xv[q] = ldf<T>(xl, o[q]);
+#pragma unroll
+                    for (int q = 0; q < 8; q++) acc = fmaf(ww[q], xv[q], acc);
+                }
+            }
+        }
+    }
+    double r2 = 0.0;
+    if (lane_ok) {
+        const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.nu + i;
+        const float cc = ag.c[nidx];
+        float sv = acc;
+        if (cc != 0.0f) sv = sv / cc;
+        const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), nidx) - sv;
+        stf<T>(reinterpret_cast<T *>(ag.r), nidx, rv);
+        const double rs = (double)ldf<T>(reinterpret_cast<const T *>(ag.r), nidx);
+        r2 = rs * rs;
+    }
+    __shared__ double sh[4];
+    const double tsum = block_sum<256>(r2, sh);
+    if (threadIdx.x == 0) ag.partial[blockIdx.x] = tsum;
+}
+
+// ------------------------------------------------------------------------
 // Back-projection, voxel-driven gather with LDS-staged projection rows.
 //
 // Block: 64 (X) x 4 (Y) voxel columns of TZ voxels each; blockIdx.y is the
@@ -993,6 +1104,25 @@ hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, i
     if (prec == MTR_PREC_FP16)
         return launch_bp_t<__half>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
     return launch_bp_t<float>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
+}
+
+int fp_stream_blocks(const AgentDev &ag) { return ((ag.nu + 3) / 4) * ((ag.nv + 63) / 64) * ag.A; }
+
+template <class T>
+static hipError_t launch_fp_stream_t(const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
+{
+    dim3 grid(max_blocks, nag), block(256);
+    if (nc == 2) hipLaunchKernelGGL((k_fp_stream<T, 2>), grid, block, 0, st, d_ags);
+    else if (nc == 3) hipLaunchKernelGGL((k_fp_stream<T, 3>), grid, block, 0, st, d_ags);
+    else hipLaunchKernelGGL((k_fp_stream<T, 4>), grid, block, 0, st, d_ags);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_fp_stream_t<__hip_bfloat16>(d_ags, nag, max_blocks, nc, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_stream_t<__half>(d_ags, nag, max_blocks, nc, st);
+    return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, st);
 }
 
 constexpr int kSepRW = 16, kSepPitch = 80;
