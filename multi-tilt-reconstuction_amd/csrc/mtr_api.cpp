@@ -155,17 +155,42 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc)
 }
 
 // Forward-projection streaming mode: every view row-separable along the
-// same axis W, W contiguous in memory (W = X), u independent of W; returns
-// the candidate count to instantiate (0 = not applicable).
-int fp_stream_nc(const std::vector<FpPlan> &plan, const double *xc)
+// same axis W with u independent of W.  Such an agent keeps its working
+// volume with W as the contiguous axis, so the 64 lanes of a wave (64
+// adjacent planes) read one coalesced row.  Returns the candidate count to
+// instantiate (0 = not applicable) and the plane axis.
+int fp_stream_nc(const std::vector<FpPlan> &plan, const double *xc, int *W_out)
 {
-    if (plan.empty() || plan[0].kind != 1 || plan[0].W != 0) return 0;
+    if (plan.empty() || plan[0].kind != 1) return 0;
+    const int W = plan[0].W;
     int nc = 0;
     for (size_t v = 0; v < plan.size(); v++) {
-        if (plan[v].kind != 1 || plan[v].W != 0 || xc[4 * v + 1] != 0.0) return 0;
+        if (plan[v].kind != 1 || plan[v].W != W || xc[4 * v + 1 + W] != 0.0) return 0;
         nc = std::max(nc, plan[v].ncand);
     }
-    return (nc >= 2 && nc <= 4) ? nc : 0;
+    if (nc < 2 || nc > 4) return 0;
+    *W_out = W;
+    return nc;
+}
+
+// Element strides of a volume whose contiguous axis is `fast` (-1 or 0:
+// the reference's [Z][Y][X]); the other two axes keep X-before-Y-before-Z
+// order.
+void layout_strides(int fast, int X, int Y, int Z, int64_t *st)
+{
+    if (fast == 1) {  // [Z][X][Y]
+        st[1] = 1;
+        st[0] = Y;
+        st[2] = (int64_t)X * Y;
+    } else if (fast == 2) {  // [Y][X][Z]
+        st[2] = 1;
+        st[0] = Z;
+        st[1] = (int64_t)X * Z;
+    } else {  // [Z][Y][X]
+        st[0] = 1;
+        st[1] = X;
+        st[2] = (int64_t)X * Y;
+    }
 }
 
 struct DevBuf {
@@ -186,6 +211,9 @@ struct Agent {
     BpMode bpm;            // back-projection kernel choice
     int fp_nc = 0;         // > 0: streaming forward projection with this candidate count
     int npartial = 0;      // per-block partials written by the last FP launch
+    int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
+    int64_t st[3] = {0, 0, 0};
+    void *d_tmp = nullptr; // canonical copy of the working volume (fast != 0)
     ViewCoef *d_coef = nullptr;
     FpPlan *d_plan = nullptr;
     void *d_p = nullptr;
@@ -314,6 +342,9 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.r = ag.d_r;
     d.x = ag.d_x;
     d.partial = ag.d_partial;
+    d.sx = ag.st[0];
+    d.sy = ag.st[1];
+    d.sz = ag.st[2];
     return d;
 }
 
@@ -487,7 +518,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     auto undo = [&](int rc) {
         Agent &g = ctx->agents.back();
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
-                        g.d_r, g.d_x, g.d_w, (void *)g.d_partial, (void *)g.d_scalar})
+                        g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -501,7 +532,10 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     ag.lanes_k = all_wx;
     ag.bpm = bp_mode(plan, xc);
-    ag.fp_nc = fp_stream_nc(plan, xc);
+    int sW = 0;
+    ag.fp_nc = fp_stream_nc(plan, xc, &sW);
+    ag.fast = ag.fp_nc ? sW : 0;
+    layout_strides(ag.fast, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st);
     const size_t es = elem_size(ctx->precision);
     const int64_t P = ag.P();
 #define ALLOC(ptr, bytes)                                                                 \
@@ -517,6 +551,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ALLOC(ag.d_r, es * P);
     ALLOC(ag.d_x, es * ctx->N);
     ALLOC(ag.d_w, es * ctx->N);
+    if (ag.fast != 0) ALLOC(ag.d_tmp, es * ctx->N);
     ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
     ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag))), 1));
     ALLOC(ag.d_scalar, sizeof(double));
@@ -623,8 +658,37 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     d.r = t.r.p;
     d.x = t.x.p;
     d.partial = nullptr;
+    d.sx = 1;
+    d.sy = vol->X;
+    d.sz = (int64_t)vol->X * vol->Y;
     HIP_TRY(hipMemcpyAsync(t.tab.p, &d, sizeof(AgentDev), hipMemcpyHostToDevice, ctx->stream));
     t.nfp = fp_blocks(d, t.lanes_k);
+    return MTR_OK;
+}
+
+// Canonical ([Z][Y][X]) view of an agent's working volume: the volume
+// itself, or its canonical copy for agents stored with another fast axis.
+void *work_canon(Agent &ag) { return ag.fast ? ag.d_tmp : ag.d_x; }
+
+int to_canon(mtr_ctx *ctx, Agent &ag)
+{
+    if (!ag.fast) return MTR_OK;
+    int64_t cs[3];
+    layout_strides(0, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, cs);
+    Timed t(ctx, MTR_K_ELEMENTWISE);
+    HIP_TRY(launch_permute(ctx->precision, ag.d_x, ag.d_tmp, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st, cs,
+                           ctx->stream));
+    return MTR_OK;
+}
+
+int from_canon(mtr_ctx *ctx, Agent &ag)
+{
+    if (!ag.fast) return MTR_OK;
+    int64_t cs[3];
+    layout_strides(0, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, cs);
+    Timed t(ctx, MTR_K_ELEMENTWISE);
+    HIP_TRY(launch_permute(ctx->precision, ag.d_tmp, ag.d_x, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, cs, ag.st,
+                           ctx->stream));
     return MTR_OK;
 }
 
@@ -632,7 +696,7 @@ void *vol_ptr(mtr_ctx *ctx, int which, int agent)
 {
     if (which == MTR_VOL_WBAR) return ctx->d_wbar;
     Agent &ag = ctx->agents[agent];
-    return which == MTR_VOL_WORK ? ag.d_x : ag.d_w;
+    return which == MTR_VOL_WORK ? work_canon(ag) : ag.d_w;
 }
 
 int copy_volume_out(mtr_ctx *ctx, const void *dev, float *host)
@@ -704,7 +768,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ag : ctx->agents) {
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
-                        ag.d_x, ag.d_w, (void *)ag.d_partial, (void *)ag.d_scalar})
+                        ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
@@ -781,7 +845,10 @@ int mtr_bpt(mtr_ctx *ctx, int agent, float *out_host)
     MTR_TRY(check_agent(ctx, agent));
     MTR_TRY(set_device(ctx));
     MTR_TRY(bp_group(ctx, agent, 1, true, 1.0f));
-    if (out_host) MTR_TRY(copy_volume_out(ctx, ctx->agents[agent].d_x, out_host));
+    if (out_host) {
+        MTR_TRY(to_canon(ctx, ctx->agents[agent]));
+        MTR_TRY(copy_volume_out(ctx, work_canon(ctx->agents[agent]), out_host));
+    }
     return collect_timing(ctx);
 }
 
@@ -790,9 +857,16 @@ int mtr_sirt(mtr_ctx *ctx, int agent, const float *x0_host, int iters, float lam
     MTR_TRY(check_agent(ctx, agent));
     if (iters < 0) return fail(MTR_ERR_ARG, "iters < 0");
     MTR_TRY(set_device(ctx));
-    if (x0_host) MTR_TRY(copy_volume_in(ctx, ctx->agents[agent].d_x, x0_host));
+    Agent &ag = ctx->agents[agent];
+    if (x0_host) {
+        MTR_TRY(copy_volume_in(ctx, work_canon(ag), x0_host));
+        MTR_TRY(from_canon(ctx, ag));
+    }
     MTR_TRY(sirt_fast(ctx, {agent}, iters, lambda));
-    if (out_host) MTR_TRY(copy_volume_out(ctx, ctx->agents[agent].d_x, out_host));
+    if (out_host) {
+        MTR_TRY(to_canon(ctx, ag));
+        MTR_TRY(copy_volume_out(ctx, work_canon(ag), out_host));
+    }
     return collect_timing(ctx);
 }
 
@@ -803,8 +877,9 @@ int mtr_form_z(mtr_ctx *ctx, int agent)
     Agent &ag = ctx->agents[agent];
     {
         Timed t(ctx, MTR_K_ELEMENTWISE);
-        HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, ag.d_x, ctx->stream));
+        HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, work_canon(ag), ctx->stream));
     }
+    MTR_TRY(from_canon(ctx, ag));
     return collect_timing(ctx);
 }
 
@@ -813,9 +888,10 @@ int mtr_mann(mtr_ctx *ctx, int agent, float rho)
     MTR_TRY(check_agent(ctx, agent));
     MTR_TRY(set_device(ctx));
     Agent &ag = ctx->agents[agent];
+    MTR_TRY(to_canon(ctx, ag));
     {
         Timed t(ctx, MTR_K_ELEMENTWISE);
-        HIP_TRY(launch_mann_z(ctx->precision, ctx->N, ag.d_x, ag.d_w, ctx->d_wbar, rho, ctx->stream));
+        HIP_TRY(launch_mann_z(ctx->precision, ctx->N, work_canon(ag), ag.d_w, ctx->d_wbar, rho, ctx->stream));
     }
     return collect_timing(ctx);
 }
@@ -834,24 +910,33 @@ int mtr_mean(mtr_ctx *ctx)
     return collect_timing(ctx);
 }
 
-int mtr_mace_init(mtr_ctx *ctx)
+// B = sum_a BPT_a into w-bar and every w_a, first working volumes z_a = B.
+// The working volumes must already hold BPT_a (agent layout).
+static int init_from_bpt(mtr_ctx *ctx)
 {
-    if (!ctx) return fail(MTR_ERR_ARG, "null context");
-    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
-    MTR_TRY(set_device(ctx));
     const int K = (int)ctx->agents.size();
-    // BPT of every agent into its working volume, one batched launch.
-    MTR_TRY(bp_group(ctx, 0, K, true, 1.0f));
     std::vector<void *> w, x;
     for (auto &ag : ctx->agents) {
+        MTR_TRY(to_canon(ctx, ag));
         w.push_back(ag.d_w);
-        x.push_back(ag.d_x);
+        x.push_back(work_canon(ag));
     }
     {
         Timed t(ctx, MTR_K_ELEMENTWISE);
         HIP_TRY(launch_init_sum(ctx->precision, K, ctx->N, w.data(), x.data(), ctx->d_wbar, ctx->stream));
     }
+    for (auto &ag : ctx->agents) MTR_TRY(from_canon(ctx, ag));
     return MTR_OK;
+}
+
+int mtr_mace_init(mtr_ctx *ctx)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    MTR_TRY(set_device(ctx));
+    // BPT of every agent into its working volume, batched per kernel kind.
+    MTR_TRY(bp_group(ctx, 0, (int)ctx->agents.size(), true, 1.0f));
+    return init_from_bpt(ctx);
 }
 
 int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form, float *wbar_host)
@@ -870,31 +955,27 @@ int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form
             HIP_TRY(launch_plain_bp(make_dev(ctx, ctx->agents[a]), (const float *)ctx->agents[a].d_p, 1.0f, 1,
                                     ctx->stream));
         }
-        std::vector<void *> w, x;
-        for (auto &ag : ctx->agents) {
-            w.push_back(ag.d_w);
-            x.push_back(ag.d_x);
-        }
-        {
-            Timed t(ctx, MTR_K_ELEMENTWISE);
-            HIP_TRY(launch_init_sum(ctx->precision, K, ctx->N, w.data(), x.data(), ctx->d_wbar, ctx->stream));
-        }
+        MTR_TRY(init_from_bpt(ctx));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<const void *> wc;
+        for (auto &ag : ctx->agents) wc.push_back(ag.d_w);
         for (int m = 0; m < M; m++) {
             for (int a = 0; a < K; a++) {
                 Agent &ag = ctx->agents[a];
                 {
                     Timed t(ctx, MTR_K_ELEMENTWISE);
-                    HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, ag.d_x, ctx->stream));
+                    HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, work_canon(ag), ctx->stream));
                 }
+                MTR_TRY(from_canon(ctx, ag));
                 MTR_TRY(sirt_plain(ctx, a, S, lambda));
+                MTR_TRY(to_canon(ctx, ag));
                 {
                     Timed t(ctx, MTR_K_ELEMENTWISE);
-                    HIP_TRY(launch_mann_z(ctx->precision, ctx->N, ag.d_x, ag.d_w, ctx->d_wbar, rho, ctx->stream));
+                    HIP_TRY(launch_mann_z(ctx->precision, ctx->N, work_canon(ag), ag.d_w, ctx->d_wbar, rho,
+                                          ctx->stream));
                 }
                 HIP_TRY(hipStreamSynchronize(ctx->stream));
             }
-            std::vector<const void *> wc(w.begin(), w.end());
             {
                 Timed t(ctx, MTR_K_ELEMENTWISE);
                 HIP_TRY(launch_mean(ctx->precision, K, ctx->N, wc.data(), ctx->d_wbar, ctx->stream));
@@ -908,14 +989,18 @@ int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form
         std::vector<void *> w, x;
         for (auto &ag : ctx->agents) {
             w.push_back(ag.d_w);
-            x.push_back(ag.d_x);
+            x.push_back(work_canon(ag));
         }
         const int cb = consensus_blocks(ctx->N);
         for (int m = 0; m < M; m++) {
             MTR_TRY(sirt_fast(ctx, ids, S, lambda));
-            Timed t(ctx, MTR_K_CONSENSUS);
-            HIP_TRY(launch_consensus(ctx->precision, K, ctx->N, rho, 1, w.data(), x.data(), ctx->d_wbar,
-                                     ctx->d_cons_partial, cb, ctx->stream));
+            for (auto &ag : ctx->agents) MTR_TRY(to_canon(ctx, ag));
+            {
+                Timed t(ctx, MTR_K_CONSENSUS);
+                HIP_TRY(launch_consensus(ctx->precision, K, ctx->N, rho, 1, w.data(), x.data(), ctx->d_wbar,
+                                         ctx->d_cons_partial, cb, ctx->stream));
+            }
+            for (auto &ag : ctx->agents) MTR_TRY(from_canon(ctx, ag));
         }
         ctx->cons_blocks = M > 0 ? cb : 0;
     }
@@ -930,6 +1015,7 @@ int mtr_get_volume(mtr_ctx *ctx, int which, int agent, float *host)
     if (which != MTR_VOL_WBAR) MTR_TRY(check_agent(ctx, agent));
     if (which < 0 || which > 2) return fail(MTR_ERR_ARG, "unknown volume %d", which);
     MTR_TRY(set_device(ctx));
+    if (which == MTR_VOL_WORK) MTR_TRY(to_canon(ctx, ctx->agents[agent]));
     return copy_volume_out(ctx, vol_ptr(ctx, which, agent), host);
 }
 
@@ -940,7 +1026,9 @@ int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host)
     if (which != MTR_VOL_WBAR) MTR_TRY(check_agent(ctx, agent));
     if (which < 0 || which > 2) return fail(MTR_ERR_ARG, "unknown volume %d", which);
     MTR_TRY(set_device(ctx));
-    return copy_volume_in(ctx, vol_ptr(ctx, which, agent), host);
+    MTR_TRY(copy_volume_in(ctx, vol_ptr(ctx, which, agent), host));
+    if (which == MTR_VOL_WORK) MTR_TRY(from_canon(ctx, ctx->agents[agent]));
+    return MTR_OK;
 }
 
 int mtr_wbar_device(mtr_ctx *ctx, void **dev_ptr)

@@ -48,12 +48,13 @@ struct AgentDev {
     int nu, nv, A;
     int Xs, Xe, Ys, Ye, Zs, Ze;  // volume box (absolute voxel coordinates)
     int X, Y, Z;
+    int64_t sx, sy, sz;    // element strides of the working volume x
     const ViewCoef *coef;  // [A]
     const FpPlan *plan;    // [A]
     const void *p;         // projections [A][nv][nu] (storage precision)
     const float *c;        // FP normaliser [A][nv][nu] (FP32 always)
     void *r;               // residual [A][nv][nu]
-    void *x;               // working volume [Z][Y][X]
+    void *x;               // working volume, strides sx/sy/sz
     double *partial;       // per-block FP64 partial sums
 };
 

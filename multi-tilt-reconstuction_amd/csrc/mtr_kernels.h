@@ -41,6 +41,10 @@ hipError_t launch_residual(int prec, int64_t P, const void *p, const float *s, c
 hipError_t launch_to_storage(int prec, int64_t n, const float *src, void *dst, hipStream_t st);
 hipError_t launch_from_storage(int prec, int64_t n, const void *src, float *dst, hipStream_t st);
 hipError_t launch_fill(int prec, int64_t n, void *dst, float v, hipStream_t st);
+// dst (strides ds) = src (strides ss) for an nX x nY x nZ volume whose
+// contiguous axes differ.
+hipError_t launch_permute(int prec, const void *src, void *dst, int nX, int nY, int nZ, const int64_t *ss,
+                          const int64_t *ds, hipStream_t st);
 hipError_t launch_reduce(const double *part, int64_t n, double *out, hipStream_t st);
 hipError_t launch_plain_splat(const AgentDev &ag, float *s, float *c, hipStream_t st);
 hipError_t launch_plain_bp(const AgentDev &ag, const float *r, float lambda, int bpt, hipStream_t st);

@@ -130,10 +130,9 @@ __device__ FpAcc fp_ray_general(const AgentDev &ag, const ViewCoef &vc, const Fp
     }
     if (!(dlo <= dhi)) return acc;
     const int d0 = max(Dlo, (int)floor(dlo)), d1 = min(Dhi, (int)ceil(dhi));
-    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
-    const int64_t sD = sel3<int64_t>(1, sy, sz, pl.D), sP = sel3<int64_t>(1, sy, sz, pl.P),
-                  sQ = sel3<int64_t>(1, sy, sz, pl.Q);
-    const int64_t base = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz;
+    const int64_t sD = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.D), sP = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.P),
+                  sQ = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.Q);
+    const int64_t base = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
     const double *cu = vc.c, *cv = vc.c + 4;
     for (int d = d0; d <= d1; d++) {
         const double pc = ps + d * pl.pd, qc = qs + d * pl.qd;
@@ -259,10 +258,9 @@ __device__ FpAcc fp_ray_separable(const AgentDev &ag, const ViewCoef &vc, const 
     }
     if (!(llo <= lhi)) return acc;
     const int l0 = max(Llo, (int)floor(llo)), l1 = min(Lhi, (int)ceil(lhi));
-    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
-    const int64_t sW = sel3<int64_t>(1, sy, sz, pl.W), sL = sel3<int64_t>(1, sy, sz, pl.L),
-                  sM = sel3<int64_t>(1, sy, sz, pl.M);
-    const int64_t base = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz + w * sW;
+    const int64_t sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W), sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L),
+                  sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M);
+    const int64_t base = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz + w * sW;
     const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
     const float aMf = (float)aM;
     const unsigned mspan = (unsigned)(Mhi - Mlo);
@@ -358,10 +356,9 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const int Wlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.W), Whi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.W) - 1;
     const int w = k * pl.wsign + (int)pl.woff;
     const bool lane_ok = i < ag.nu && k < ag.nv && w >= Wlo && w <= Whi;
-    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
-    const int64_t sL = sel3<int64_t>(1, sy, sz, pl.L), sM = sel3<int64_t>(1, sy, sz, pl.M),
-                  sW = sel3<int64_t>(1, sy, sz, pl.W);
-    const int64_t base0 = -(int64_t)ag.Xs - (int64_t)ag.Ys * sy - (int64_t)ag.Zs * sz;
+    const int64_t sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L), sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M),
+                  sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W);
+    const int64_t base0 = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
     const T *xl = x + base0 + (lane_ok ? w : Wlo) * sW;  // this lane's plane
     float acc = 0.f;
     if (i < ag.nu) {
@@ -581,8 +578,8 @@ __global__ __launch_bounds__(256) void k_bp(const AgentDev *__restrict__ ags, fl
     }
     if (!active) return;
     T *x = reinterpret_cast<T *>(ag.x);
-    const int64_t sxy = (int64_t)ag.X * ag.Y;
-    const int64_t j0 = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X + (int64_t)(Z0 - ag.Zs) * sxy;
+    const int64_t sxy = ag.sz;
+    const int64_t j0 = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy + (int64_t)(Z0 - ag.Zs) * sxy;
 #pragma unroll
     for (int t = 0; t < TZ; t++) {
         if (t > ez) break;
@@ -735,9 +732,8 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     }
     if (!active) return;
     T *x = reinterpret_cast<T *>(ag.x);
-    const int64_t sy = ag.X, sz = (int64_t)ag.X * ag.Y;
-    const int64_t sW = sel3<int64_t>(1, sy, sz, sa.W);
-    const int64_t j0 = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * sy + (int64_t)(Z - ag.Zs) * sz;
+    const int64_t sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, sa.W);
+    const int64_t j0 = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy + (int64_t)(Z - ag.Zs) * ag.sz;
 #pragma unroll
     for (int t = 0; t < RW; t++) {
         if (t > ew) break;
@@ -933,6 +929,45 @@ __global__ void k_fill(int64_t n, T *__restrict__ dst, float v)
         stf<T>(dst, j, v);
 }
 
+// Volume layout change: element (X, Y, Z) moves from src[X*ss.x + Y*ss.y +
+// Z*ss.z] to dst[X*ds.x + Y*ds.y + Z*ds.z].  A is the axis contiguous in
+// src, B the axis contiguous in dst; 32 x 32 tiles of the (A, B) plane
+// pass through LDS so that both the reads and the writes are coalesced.
+struct PermArgs {
+    int n[3];
+    int64_t ss[3], ds[3];
+    int A, B, C;
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_permute(const T *__restrict__ src, T *__restrict__ dst, PermArgs pa)
+{
+    __shared__ float tile[32][33];
+    const int nA = pa.n[pa.A], nB = pa.n[pa.B], nC = pa.n[pa.C];
+    const int ta = (nA + 31) / 32, tb = (nB + 31) / 32;
+    int64_t b = blockIdx.x;
+    if (b >= (int64_t)ta * tb * nC) return;
+    const int ia = (int)(b % ta);
+    b /= ta;
+    const int ib = (int)(b % tb);
+    const int c = (int)(b / tb);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int a0 = ia * 32, b0 = ib * 32;
+    const int64_t sA = pa.ss[pa.A], sB = pa.ss[pa.B], sC = pa.ss[pa.C];
+    const int64_t dA = pa.ds[pa.A], dB = pa.ds[pa.B], dC = pa.ds[pa.C];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int bb = b0 + ty + 8 * q, aa = a0 + tx;
+        if (aa < nA && bb < nB) tile[ty + 8 * q][tx] = ldf<T>(src, aa * sA + bb * sB + c * sC);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int aa = a0 + ty + 8 * q, bb = b0 + tx;
+        if (aa < nA && bb < nB) stf<T>(dst, aa * dA + bb * dB + c * dC, tile[tx][ty + 8 * q]);
+    }
+}
+
 // Deterministic fixed-order sum of n FP64 partials (one block of 256).
 __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int64_t n,
                                                double *__restrict__ out)
@@ -962,8 +997,8 @@ __global__ __launch_bounds__(512) void k_plain_splat(AgentDev ag, float *__restr
     const int Z = ag.Zs + blockIdx.z * 8 + threadIdx.z;
     if (X >= ag.Xe || Y >= ag.Ye || Z >= ag.Ze) return;
     const float *x = reinterpret_cast<const float *>(ag.x);
-    const double xv = (double)x[(int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X +
-                                (int64_t)(Z - ag.Zs) * ag.X * ag.Y];
+    const double xv = (double)x[(int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy +
+                                (int64_t)(Z - ag.Zs) * ag.sz];
     const int nu = ag.nu, nv = ag.nv;
     for (int a = 0; a < ag.A; a++) {
         const ViewCoef vc = ag.coef[a];
@@ -1014,7 +1049,7 @@ __global__ __launch_bounds__(512) void k_plain_bp(AgentDev ag, const float *__re
         }
     }
     float *x = reinterpret_cast<float *>(ag.x);
-    const int64_t j = (int64_t)(X - ag.Xs) + (int64_t)(Y - ag.Ys) * ag.X + (int64_t)(Z - ag.Zs) * ag.X * ag.Y;
+    const int64_t j = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy + (int64_t)(Z - ag.Zs) * ag.sz;
     if (bpt) {
         x[j] = C != 0.0 ? (float)(S / C) : 0.0f;
     } else if (C != 0.0) {
@@ -1312,6 +1347,35 @@ hipError_t launch_fill(int prec, int64_t n, void *dst, float v, hipStream_t st)
         hipLaunchKernelGGL(k_fill<__half>, dim3(ew_grid(n)), dim3(256), 0, st, n, (__half *)dst, v);
     else
         hipLaunchKernelGGL(k_fill<float>, dim3(ew_grid(n)), dim3(256), 0, st, n, (float *)dst, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute(int prec, const void *src, void *dst, int nX, int nY, int nZ, const int64_t *ss,
+                          const int64_t *ds, hipStream_t st)
+{
+    PermArgs pa;
+    pa.n[0] = nX;
+    pa.n[1] = nY;
+    pa.n[2] = nZ;
+    for (int q = 0; q < 3; q++) {
+        pa.ss[q] = ss[q];
+        pa.ds[q] = ds[q];
+    }
+    pa.A = ss[0] == 1 ? 0 : (ss[1] == 1 ? 1 : 2);
+    pa.B = ds[0] == 1 ? 0 : (ds[1] == 1 ? 1 : 2);
+    if (pa.A == pa.B) return hipErrorInvalidValue;  // same fast axis: not a transpose
+    pa.C = 3 - pa.A - pa.B;
+    const int64_t nb = (int64_t)((pa.n[pa.A] + 31) / 32) * ((pa.n[pa.B] + 31) / 32) * pa.n[pa.C];
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    if (prec == MTR_PREC_BF16)
+        hipLaunchKernelGGL(k_permute<__hip_bfloat16>, dim3((unsigned)nb), dim3(256), 0, st,
+                           (const __hip_bfloat16 *)src, (__hip_bfloat16 *)dst, pa);
+    else if (prec == MTR_PREC_FP16)
+        hipLaunchKernelGGL(k_permute<__half>, dim3((unsigned)nb), dim3(256), 0, st, (const __half *)src, (__half *)dst,
+                           pa);
+    else
+        hipLaunchKernelGGL(k_permute<float>, dim3((unsigned)nb), dim3(256), 0, st, (const float *)src, (float *)dst,
+                           pa);
     return hipGetLastError();
 }
 

@@ -276,3 +276,18 @@ def test_c2_full_solve_properties(mtr):
         r2 = [c.residual_norm2(a) for a in range(pb.K)]
     assert np.isfinite(out).all() and np.isfinite(d2) and all(np.isfinite(r2))
     assert float(np.abs(out).max()) > 0
+
+
+@pytest.mark.parametrize("phi,kind", [(0.0, 0), (90.0, 0), (0.0, 1), (45.0, 0)], ids=["W=Y", "W=X", "W=Z", "general"])
+def test_working_volume_layout_round_trip(mtr, phi, kind):
+    # agents keep their working volume with the plane axis contiguous; the
+    # API always exchanges the reference's [Z][Y][X] layout
+    pb = problem(mtr, [phi], kind=kind, X=37, Y=29, Z=21, views=5, seed=31)
+    rng = np.random.default_rng(7)
+    v = rng.normal(size=(pb.vol.Z, pb.vol.Y, pb.vol.X)).astype(np.float32)
+    say("layout round trip", phi, kind)
+    with load_ctx(mtr, pb) as c:
+        c.set_volume_data(mtr.VOL_WORK, 0, v)
+        assert np.array_equal(c.get_volume(mtr.VOL_WORK, 0), v)
+        out = c.sirt(0, 0, 0.5, x0=v)
+        assert np.array_equal(out, v)
