@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -133,9 +134,11 @@ int make_plan(const double *xc, const double *yc, FpPlan &pl)
 struct BpMode {
     bool sep = false;
     int W = 0, P1 = 0, P2 = 0;
+    int rw = 32;  // voxels per thread along W
 };
 
-BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc)
+BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc, const double *yc, const mtr_volume_size &vol,
+                const mtr_projection_size &prj)
 {
     BpMode m;
     if (plan.empty()) return m;
@@ -143,9 +146,13 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc)
     for (size_t v = 0; v < plan.size(); v++)
         if (plan[v].kind != 1 || plan[v].W != W || xc[4 * v + 1 + W] != 0.0) return m;
     const int P1 = W == 0 ? 1 : 0, P2 = 3 - W - P1;
+    const int lo[3] = {vol.Xstart, vol.Ystart, vol.Zstart}, hi[3] = {vol.Xend - 1, vol.Yend - 1, vol.Zend - 1};
     for (size_t v = 0; v < plan.size(); v++) {
         const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * 3 + 5;
         if (w > bp_sep_pitch()) return m;
+        // every plane of the box must land on a detector row
+        const double k0 = yc[4 * v] + yc[4 * v + 1 + W] * lo[W] - 1, k1 = yc[4 * v] + yc[4 * v + 1 + W] * hi[W] - 1;
+        if (std::min(k0, k1) < 0 || std::max(k0, k1) > prj.Y - 1) return m;
     }
     m.sep = true;
     m.W = W;
@@ -422,8 +429,8 @@ int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
         const AgentDev d = make_dev(ctx, ctx->agents[s]);
         Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
         if (m.sep) {
-            HIP_TRY(launch_bp_sep(ctx->precision, ctx->d_ags + s, e - s, bp_sep_blocks(d, m.W, m.P1, m.P2), bpt, lambda,
-                                  m.W, m.P1, m.P2, ctx->stream));
+            HIP_TRY(launch_bp_sep(ctx->precision, ctx->d_ags + s, e - s, bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw), bpt,
+                                  lambda, m.W, m.P1, m.P2, m.rw, ctx->stream));
         } else {
             HIP_TRY(launch_bp(ctx->precision, ctx->d_ags + s, e - s, bp_blocks(d, ctx->bp_tz), ctx->bp_tz, bpt, lambda,
                               nullptr, ctx->lds_cap, ctx->stream));
@@ -531,7 +538,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     bool all_wx = true;
     for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     ag.lanes_k = all_wx;
-    ag.bpm = bp_mode(plan, xc);
+    ag.bpm = bp_mode(plan, xc, yc, ctx->vol, *prj);
+    if (const char *e = std::getenv("MTR_BP_RW")) ag.bpm.rw = std::atoi(e) == 8 ? 8 : (std::atoi(e) == 16 ? 16 : 32);
     int sW = 0;
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     ag.fast = ag.fp_nc ? sW : 0;
@@ -626,7 +634,7 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
         all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     }
     t.lanes_k = all_wx;
-    t.bpm = bp_mode(plan, xc);
+    t.bpm = bp_mode(plan, xc, yc, *vol, *prj);
     const int64_t P = (int64_t)prj->X * prj->Y * A;
     const int64_t N = (int64_t)vol->X * vol->Y * vol->Z;
     HIP_TRY(hipMalloc(&t.coef.p, sizeof(ViewCoef) * A));
@@ -1171,8 +1179,8 @@ static int op_bp_common(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_
         Timed tm(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
         if (t.bpm.sep)
             HIP_TRY(launch_bp_sep(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1,
-                                  bp_sep_blocks(t.dev, t.bpm.W, t.bpm.P1, t.bpm.P2), bpt, lambda, t.bpm.W, t.bpm.P1,
-                                  t.bpm.P2, ctx->stream));
+                                  bp_sep_blocks(t.dev, t.bpm.W, t.bpm.P1, t.bpm.P2, t.bpm.rw), bpt, lambda, t.bpm.W,
+                                  t.bpm.P1, t.bpm.P2, t.bpm.rw, ctx->stream));
         else
             HIP_TRY(launch_bp(MTR_PREC_FP32, (const AgentDev *)t.tab.p, 1, bp_blocks(t.dev, ctx->bp_tz), ctx->bp_tz,
                               bpt, lambda, t.r.p, ctx->lds_cap, ctx->stream));

@@ -13,10 +13,11 @@ int consensus_blocks(int64_t N);
 int fp_stream_blocks(const AgentDev &ag);
 hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st);
 // Row-separable back-projection (runs along the plane axis W).
-int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2);
+// rw: voxels per thread along W (8, 16 or 32).
+int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw);
 int bp_sep_pitch();
 hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
-                         int P1, int P2, hipStream_t st);
+                         int P1, int P2, int rw, hipStream_t st);
 
 // Forward projection of nag agents (device table d_ags), mode 0 = fused
 // residual, 1 = raw s/c into s_out/c_out, 2 = normaliser into c_out.

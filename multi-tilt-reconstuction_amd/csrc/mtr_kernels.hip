@@ -48,6 +48,35 @@ __device__ __forceinline__ void stf<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i
 template <>
 __device__ __forceinline__ void stf<__half>(__half *p, int64_t i, float v) { p[i] = __float2half(v); }
 
+// Global (address space 1) view of a pointer held in a struct: lets the
+// compiler emit global loads with a scalar base instead of flat loads.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gview(const T *p)
+{
+    return (const __attribute__((address_space(1))) T *)p;
+}
+
+template <class T>
+__device__ __forceinline__ float ldg(const __attribute__((address_space(1))) T *p, uint32_t i);
+template <>
+__device__ __forceinline__ float ldg<float>(const __attribute__((address_space(1))) float *p, uint32_t i)
+{
+    return p[i];
+}
+template <>
+__device__ __forceinline__ float ldg<__hip_bfloat16>(const __attribute__((address_space(1))) __hip_bfloat16 *p,
+                                                     uint32_t i)
+{
+    const unsigned short b = ((const __attribute__((address_space(1))) unsigned short *)p)[i];
+    return __uint_as_float((unsigned)b << 16);
+}
+template <>
+__device__ __forceinline__ float ldg<__half>(const __attribute__((address_space(1))) __half *p, uint32_t i)
+{
+    const unsigned short b = ((const __attribute__((address_space(1))) unsigned short *)p)[i];
+    return __half2float(__ushort_as_half(b));
+}
+
 // The reference's coordinate chain, left to right with fused multiply-adds.
 __device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
 {
@@ -359,7 +388,9 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const int64_t sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L), sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M),
                   sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W);
     const int64_t base0 = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
-    const T *xl = x + base0 + (lane_ok ? w : Wlo) * sW;  // this lane's plane
+    // uniform base (first plane of the box) + per-lane 32-bit plane offset
+    const auto xb = gview(x + base0 + (int64_t)Wlo * sW);
+    const uint32_t lane_off = (uint32_t)(((lane_ok ? w : Wlo) - Wlo) * sW);
     float acc = 0.f;
     if (i < ag.nu) {
         const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
@@ -394,11 +425,16 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                     if (q <= j) t += aMf;
                 const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
                 const float wt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
-                const int off = ok ? (int)(l * sL + m * sM) : 0;
+                // offset from the first plane of the box, >= 0
+                const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) : 0;
+                const int64_t lm0 = (int64_t)Llo * sL + (int64_t)Mlo * sM;  // folded into the base below
+                const auto xc = xb + lm0;
                 uint64_t mask = __ballot(wt > 0.0f);
-                while (mask) {
-                    int o[8];
-                    float ww[8], xv[8];
+                // two groups of 8 taps in flight: the next group's loads are
+                // issued before the current group is accumulated
+                int oa[8], ob[8];
+                float wa[8], wb[8], xa[8], xq[8];
+                auto extract = [&](int (&o)[8], float (&ww)[8]) {
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         if (mask) {
@@ -411,10 +447,34 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                             ww[q] = 0.0f;
                         }
                     }
+                };
+                auto issue = [&](const int (&o)[8], float (&xv)[8]) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) xv[q] = ldf<T>(xl, o[q]);
+                    for (int q = 0; q < 8; q++) xv[q] = ldg<T>(xc + o[q], lane_off);
+                };
+                auto consume = [&](const float (&ww)[8], const float (&xv)[8]) {
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc = fmaf(ww[q], xv[q], acc);
+                };
+                if (mask) {
+                    extract(oa, wa);
+                    issue(oa, xa);
+                    while (true) {
+                        if (!mask) {
+                            consume(wa, xa);
+                            break;
+                        }
+                        extract(ob, wb);
+                        issue(ob, xq);
+                        consume(wa, xa);
+                        if (!mask) {
+                            consume(wb, xq);
+                            break;
+                        }
+                        extract(oa, wa);
+                        issue(oa, xa);
+                        consume(wb, xq);
+                    }
                 }
             }
         }
@@ -635,7 +695,9 @@ __device__ __forceinline__ void bp_sep_footprint(const AgentDev &ag, const ViewC
 template <class T, int RW, int PITCH, bool BPT>
 __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda, BpSepArgs sa)
 {
-    __shared__ float lds[2][RW * PITCH];
+    // LDS holds (r[k][i], r[k][i+1] - r[k][i]) pairs so that the interior
+    // update is S += r0 + du * d: one 64-bit LDS read and two FP32 ops.
+    __shared__ float2 lds[2][RW * PITCH];
     const AgentDev &ag = ags[blockIdx.y];
     const int n1 = sel3(ag.X, ag.Y, ag.Z, sa.P1), n2 = sel3(ag.X, ag.Y, ag.Z, sa.P2), nw = sel3(ag.X, ag.Y, ag.Z, sa.W);
     const int s1 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P1), s2 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P2), sw = sel3(ag.Xs, ag.Ys, ag.Zs, sa.W);
@@ -654,32 +716,34 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     const bool active = p1 < s1 + n1 && p2 < s2 + n2;
     const int nu = ag.nu, nv = ag.nv;
     const int64_t plane = (int64_t)nu * nv;
-    const T *src = reinterpret_cast<const T *>(BPT ? ag.p : ag.r);
+    const auto src = gview(reinterpret_cast<const T *>(BPT ? ag.p : ag.r));
     const int X = sa.P1 == 0 ? p1 : (sa.P2 == 0 ? p2 : W0);
     const int Y = sa.P1 == 1 ? p1 : (sa.P2 == 1 ? p2 : W0);
     const int Z = sa.P1 == 2 ? p1 : (sa.P2 == 2 ? p2 : W0);
     using St = BpSepStage<RW, PITCH>;
 
-    float S[RW], C[RW];
+    float S[RW];
 #pragma unroll
-    for (int t = 0; t < RW; t++) S[t] = C[t] = 0.f;
-    float Cint = 0.f;
-    float pre[St::kPer];
+    for (int t = 0; t < RW; t++) S[t] = 0.f;
+    float Cacc = 0.f;  // every row of the run is on the detector (host-checked)
+    float2 pre[St::kPer];
 
-    // fetch the footprint of view a into registers
     auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
         const ViewCoef vc = ag.coef[a];
         bp_sep_footprint<T, RW, PITCH>(ag, vc, sa, P10, P20, W0, e1, e2, fi0, width, k0row, kstep);
-        const T *rv = src + plane * a;
+        const uint32_t vbase = (uint32_t)(plane * a);
 #pragma unroll
         for (int q = 0; q < St::kPer; q++) {
             const int e = threadIdx.x + 256 * q;
             const int row = e / PITCH, col = e - row * PITCH;
             const int gk = k0row + kstep * row, gi = fi0 + col;
-            float val = 0.f;
-            if (e < St::kElems && col < width && row <= ew && (unsigned)gk < (unsigned)nv && (unsigned)gi < (unsigned)nu)
-                val = ldf<T>(rv, (int64_t)gk * nu + gi);
-            pre[q] = val;
+            float v0 = 0.f, v1 = 0.f;
+            if (e < St::kElems && col < width && row <= ew && (unsigned)gk < (unsigned)nv) {
+                const uint32_t o = vbase + (uint32_t)gk * nu;
+                if ((unsigned)gi < (unsigned)nu) v0 = ldg<T>(src, o + gi);
+                if ((unsigned)(gi + 1) < (unsigned)nu) v1 = ldg<T>(src, o + gi + 1);
+            }
+            pre[q] = make_float2(v0, v1 - v0);
         }
     };
     auto commit = [&](int buf) {
@@ -695,7 +759,7 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     commit(0);
     __syncthreads();
     for (int a = 0; a < ag.A; a++) {
-        const int cfi0 = fi0, cwidth = width, ck0 = k0row, cks = kstep;
+        const int cfi0 = fi0, cwidth = width;
         if (a + 1 < ag.A) fetch(a + 1, fi0, width, k0row, kstep);
         const ViewCoef vc = ag.coef[a];
         const double u = coord(vc.c, X, Y, Z);
@@ -703,29 +767,26 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
         const int i0 = (int)fu;
         const float du = (float)(u - fu);
         const int li = min(max(i0 - cfi0, 0), min(cwidth, PITCH) - 2);
-        const float *q = &lds[a & 1][li];
-        const int klast = ck0 + cks * ew;
-        const bool rows_in = (unsigned)ck0 < (unsigned)nv && (unsigned)klast < (unsigned)nv;
-        const bool interior = rows_in && cfi0 >= 0 && cfi0 + cwidth - 1 <= nu - 1;
+        const float2 *q = &lds[a & 1][li];
+        const bool interior = cfi0 >= 0 && cfi0 + cwidth - 1 <= nu - 1;
         if (cwidth < 2) {
             // the view misses this tile: no tap is on the detector
         } else if (interior) {
 #pragma unroll
             for (int t = 0; t < RW; t++) {
-                const float r0 = q[t * PITCH], r1 = q[t * PITCH + 1];
-                S[t] += fmaf(du, r1 - r0, r0);
+                const float2 rd = q[t * PITCH];
+                S[t] = fmaf(du, rd.y, S[t] + rd.x);
             }
-            Cint += 1.0f;
+            Cacc += 1.0f;
         } else {
             const float wu0 = (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
             const float wu1 = (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
 #pragma unroll
             for (int t = 0; t < RW; t++) {
-                const float r0 = q[t * PITCH], r1 = q[t * PITCH + 1];
-                const bool rv = (unsigned)(ck0 + cks * t) < (unsigned)nv;
-                S[t] += wu0 * r0 + wu1 * r1;
-                C[t] += rv ? wu0 + wu1 : 0.f;
+                const float2 rd = q[t * PITCH];
+                S[t] += wu0 * rd.x + wu1 * (rd.x + rd.y);
             }
+            Cacc += wu0 + wu1;
         }
         if (a + 1 < ag.A) commit((a + 1) & 1);
         __syncthreads();
@@ -738,11 +799,10 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     for (int t = 0; t < RW; t++) {
         if (t > ew) break;
         const int64_t j = j0 + t * sW;
-        const float Ct = Cint + C[t];
         if (BPT) {
-            stf<T>(x, j, Ct != 0.0f ? S[t] / Ct : 0.0f);
-        } else if (Ct != 0.0f) {
-            stf<T>(x, j, fmaf(S[t] / Ct, lambda, ldf<T>(x, j)));
+            stf<T>(x, j, Cacc != 0.0f ? S[t] / Cacc : 0.0f);
+        } else if (Cacc != 0.0f) {
+            stf<T>(x, j, fmaf(S[t] / Cacc, lambda, ldf<T>(x, j)));
         }
     }
 }
@@ -1160,37 +1220,45 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
     return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, st);
 }
 
-constexpr int kSepRW = 16, kSepPitch = 80;
+constexpr int kSepPitch = 72;
 
-int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2)
+int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 {
     auto ext = [&](int ax) { return ax == 0 ? ag.X : (ax == 1 ? ag.Y : ag.Z); };
-    const int nb = ((ext(P1) + 63) / 64) * ((ext(P2) + 3) / 4) * ((ext(W) + kSepRW - 1) / kSepRW);
+    const int nb = ((ext(P1) + 63) / 64) * ((ext(P2) + 3) / 4) * ((ext(W) + rw - 1) / rw);
     return ((nb + 7) / 8) * 8;
 }
 
 int bp_sep_pitch() { return kSepPitch; }
 
+template <class T, int RW>
+static void launch_bp_sep_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, BpSepArgs sa, hipStream_t st)
+{
+    if (bpt)
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, true>), grid, dim3(256), 0, st, d_ags, lambda, sa);
+    else
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, false>), grid, dim3(256), 0, st, d_ags, lambda, sa);
+}
+
 template <class T>
 static hipError_t launch_bp_sep_t(const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
-                                  int P1, int P2, hipStream_t st)
+                                  int P1, int P2, int rw, hipStream_t st)
 {
     BpSepArgs sa{W, P1, P2};
-    dim3 grid(max_blocks, nag), block(256);
-    if (bpt)
-        hipLaunchKernelGGL((k_bp_sep<T, kSepRW, kSepPitch, true>), grid, block, 0, st, d_ags, lambda, sa);
-    else
-        hipLaunchKernelGGL((k_bp_sep<T, kSepRW, kSepPitch, false>), grid, block, 0, st, d_ags, lambda, sa);
+    dim3 grid(max_blocks, nag);
+    if (rw == 32) launch_bp_sep_rw<T, 32>(grid, d_ags, bpt, lambda, sa, st);
+    else if (rw == 8) launch_bp_sep_rw<T, 8>(grid, d_ags, bpt, lambda, sa, st);
+    else launch_bp_sep_rw<T, 16>(grid, d_ags, bpt, lambda, sa, st);
     return hipGetLastError();
 }
 
 hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
-                         int P1, int P2, hipStream_t st)
+                         int P1, int P2, int rw, hipStream_t st)
 {
     if (prec == MTR_PREC_BF16)
-        return launch_bp_sep_t<__hip_bfloat16>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
-    if (prec == MTR_PREC_FP16) return launch_bp_sep_t<__half>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
-    return launch_bp_sep_t<float>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, st);
+        return launch_bp_sep_t<__hip_bfloat16>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, rw, st);
+    if (prec == MTR_PREC_FP16) return launch_bp_sep_t<__half>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, rw, st);
+    return launch_bp_sep_t<float>(d_ags, nag, max_blocks, bpt, lambda, W, P1, P2, rw, st);
 }
 
 int consensus_blocks(int64_t N) { return ew_grid(N) > 8192 ? 8192 : ew_grid(N); }
@@ -1361,9 +1429,19 @@ hipError_t launch_permute(int prec, const void *src, void *dst, int nX, int nY, 
         pa.ss[q] = ss[q];
         pa.ds[q] = ds[q];
     }
-    pa.A = ss[0] == 1 ? 0 : (ss[1] == 1 ? 1 : 2);
-    pa.B = ds[0] == 1 ? 0 : (ds[1] == 1 ? 1 : 2);
-    if (pa.A == pa.B) return hipErrorInvalidValue;  // same fast axis: not a transpose
+    // Axes of extent 1 carry no layout: compare the others.
+    bool same = true;
+    for (int q = 0; q < 3; q++)
+        if (pa.n[q] > 1 && ss[q] != ds[q]) same = false;
+    if (same)
+        return hipMemcpyAsync(dst, src, (size_t)nX * nY * nZ * (prec == MTR_PREC_FP32 ? 4 : 2),
+                              hipMemcpyDeviceToDevice, st);
+    pa.A = pa.B = -1;
+    for (int q = 0; q < 3; q++) {
+        if (pa.n[q] > 1 && ss[q] == 1 && pa.A < 0) pa.A = q;
+        if (pa.n[q] > 1 && ds[q] == 1 && pa.B < 0) pa.B = q;
+    }
+    if (pa.A < 0 || pa.B < 0 || pa.A == pa.B) return hipErrorInvalidValue;
     pa.C = 3 - pa.A - pa.B;
     const int64_t nb = (int64_t)((pa.n[pa.A] + 31) / 32) * ((pa.n[pa.B] + 31) / 32) * pa.n[pa.C];
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
