@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--calibrate-copy-mb", type=int, default=256,
+                    help="size of the streaming copy run once for traffic-counter calibration (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU agent sharding is a "
@@ -103,6 +105,11 @@ def main() -> int:
     A = pb.prjs[0].AngN
 
     ctx = mtr.Context(0, prec)
+    copy_cal = None
+    if args.calibrate_copy_mb > 0:
+        nbytes = args.calibrate_copy_mb << 20
+        ms = ctx.calibrate_copy(nbytes)
+        copy_cal = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
     ctx.set_volume(pb.vol)
     for a in range(pb.K):
         ctx.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
@@ -221,6 +228,7 @@ def main() -> int:
         "plain_loop": plain,
         "speedup_vs_plain": (value / plain["recon_per_s"]) if plain else None,
         "parity_sample": parity,
+        "copy_calibration": copy_cal,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -202,6 +202,10 @@ int mtr_timing_enable(mtr_ctx *ctx, int on);
 int mtr_timing_reset(mtr_ctx *ctx);
 /* Launch count and summed event time (ms) of one kernel class. */
 int mtr_timing_get(mtr_ctx *ctx, int kernel, int64_t *launches, double *ms);
+/* Copies `bytes` between two fresh device buffers with the library's
+ * streaming copy kernel and returns the kernel time; used to calibrate HBM
+ * traffic counters against a copy of known size. */
+int mtr_calibrate_copy(mtr_ctx *ctx, size_t bytes, double *ms);
 /* Blocks until all queued work of the context is done. */
 int mtr_sync(mtr_ctx *ctx);
 

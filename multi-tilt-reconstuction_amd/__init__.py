@@ -200,6 +200,11 @@ class Context:
         _check(lib.mtr_consensus_diff2(self._h, _ct.byref(v)), "consensus_diff2")
         return v.value
 
+    def calibrate_copy(self, nbytes: int) -> float:
+        ms = _ct.c_double()
+        _check(lib.mtr_calibrate_copy(self._h, nbytes, _ct.byref(ms)), "calibrate_copy")
+        return ms.value
+
     def sync(self) -> None:
         _check(lib.mtr_sync(self._h), "sync")
 

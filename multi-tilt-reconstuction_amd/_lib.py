@@ -78,6 +78,7 @@ _PROTOS = {
     "mtr_timing_reset": (_i, [_v]),
     "mtr_timing_get": (_i, [_v, _i, ct.POINTER(_i64), _pd]),
     "mtr_sync": (_i, [_v]),
+    "mtr_calibrate_copy": (_i, [_v, _sz, _pd]),
     # file boundary (mtr_io.h)
     "mtr_mrc_read_header": (_i, [ct.c_char_p, _v]),
     "mtr_mrc_read_data": (_i, [ct.c_char_p, _pf, _i64]),

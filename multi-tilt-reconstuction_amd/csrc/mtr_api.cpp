@@ -1242,6 +1242,30 @@ int mtr_timing_get(mtr_ctx *ctx, int kernel, int64_t *launches, double *ms)
     return MTR_OK;
 }
 
+int mtr_calibrate_copy(mtr_ctx *ctx, size_t bytes, double *ms)
+{
+    if (!ctx || !ms || bytes < 4) return fail(MTR_ERR_ARG, "bad argument");
+    MTR_TRY(set_device(ctx));
+    DevBuf a, b;
+    const int64_t n = (int64_t)(bytes / 4);
+    HIP_TRY(hipMalloc(&a.p, sizeof(float) * n));
+    HIP_TRY(hipMalloc(&b.p, sizeof(float) * n));
+    HIP_TRY(launch_fill(MTR_PREC_FP32, n, a.p, 1.0f, ctx->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, ctx->stream));
+    HIP_TRY(launch_to_storage(MTR_PREC_FP32, n, (const float *)a.p, b.p, ctx->stream));
+    HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t;
+    return MTR_OK;
+}
+
 int mtr_sync(mtr_ctx *ctx)
 {
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
