@@ -148,7 +148,7 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc, const double *
     const int P1 = W == 0 ? 1 : 0, P2 = 3 - W - P1;
     const int lo[3] = {vol.Xstart, vol.Ystart, vol.Zstart}, hi[3] = {vol.Xend - 1, vol.Yend - 1, vol.Zend - 1};
     for (size_t v = 0; v < plan.size(); v++) {
-        const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * 3 + 5;
+        const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * 7 + 5;
         if (w > bp_sep_pitch()) return m;
         // every plane of the box must land on a detector row
         const double k0 = yc[4 * v] + yc[4 * v + 1 + W] * lo[W] - 1, k1 = yc[4 * v] + yc[4 * v + 1 + W] * hi[W] - 1;

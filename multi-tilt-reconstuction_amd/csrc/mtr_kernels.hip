@@ -368,10 +368,14 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
 template <class T, int NC>
 __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ ags)
 {
+    __shared__ int2 taps[4][72];  // per wave: (offset, weight bits), 64 + 8 padding
     const AgentDev &ag = ags[blockIdx.y];
     const int nti = (ag.nu + 3) / 4, ntk = (ag.nv + 63) / 64;
-    int b = blockIdx.x;
-    if (b >= nti * ntk * ag.A) return;
+    // XCD-aware order: adjacent ray groups of one view share an XCD's L2
+    const int nbt = nti * ntk * ag.A, per = (nbt + 7) / 8;
+    if (blockIdx.x >= (unsigned)(per * 8)) return;
+    int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (b >= nbt) return;
     const int bi = b % nti;
     b /= nti;
     const int bk = b % ntk;
@@ -429,53 +433,32 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                 const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) : 0;
                 const int64_t lm0 = (int64_t)Llo * sL + (int64_t)Mlo * sM;  // folded into the base below
                 const auto xc = xb + lm0;
-                uint64_t mask = __ballot(wt > 0.0f);
-                // two groups of 8 taps in flight: the next group's loads are
-                // issued before the current group is accumulated
-                int oa[8], ob[8];
-                float wa[8], wb[8], xa[8], xq[8];
-                auto extract = [&](int (&o)[8], float (&ww)[8]) {
+                // compact the nonzero taps of this chunk into the wave's LDS
+                // list (order preserved), padded to a multiple of 8
+                const uint64_t mask = __ballot(wt > 0.0f);
+                const int cnt = __popcll(mask);
+                const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+                if (wt > 0.0f) taps[wid][pos] = make_int2(off, __float_as_int(wt));
+                if (lane >= cnt && lane < cnt + 8) taps[wid][lane] = make_int2(0, 0);
+                if (lane < 8 && 64 + lane < cnt + 8) taps[wid][64 + lane] = make_int2(0, 0);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int e = 0; e < cnt; e += 8) {
+                    float xv[8], ww[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
-                        if (mask) {
-                            const int e = __builtin_ctzll(mask);
-                            mask &= mask - 1;
-                            o[q] = __builtin_amdgcn_readlane(off, e);
-                            ww[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wt), e));
-                        } else {
-                            o[q] = 0;
-                            ww[q] = 0.0f;
-                        }
+                        const int2 tp = taps[wid][e + q];
+                        ww[q] = __int_as_float(tp.y);
+                        xv[q] = ldg<T>(xc, (uint32_t)tp.x + lane_off);
                     }
-                };
-                auto issue = [&](const int (&o)[8], float (&xv)[8]) {
-#pragma unroll
-                    for (int q = 0; q < 8; q++) xv[q] = ldg<T>(xc + o[q], lane_off);
-                };
-                auto consume = [&](const float (&ww)[8], const float (&xv)[8]) {
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc = fmaf(ww[q], xv[q], acc);
-                };
-                if (mask) {
-                    extract(oa, wa);
-                    issue(oa, xa);
-                    while (true) {
-                        if (!mask) {
-                            consume(wa, xa);
-                            break;
-                        }
-                        extract(ob, wb);
-                        issue(ob, xq);
-                        consume(wa, xa);
-                        if (!mask) {
-                            consume(wb, xq);
-                            break;
-                        }
-                        extract(oa, wa);
-                        issue(oa, xa);
-                        consume(wb, xq);
-                    }
                 }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
     }
@@ -666,10 +649,10 @@ struct BpSepArgs {
     int W, P1, P2;  // axes (0 = X, 1 = Y, 2 = Z)
 };
 
-template <int RW, int PITCH>
+template <int RW, int PITCH, int NT>
 struct BpSepStage {
     static constexpr int kElems = RW * PITCH;
-    static constexpr int kPer = (kElems + 255) / 256;
+    static constexpr int kPer = (kElems + NT - 1) / NT;
 };
 
 template <class T, int RW, int PITCH>
@@ -692,16 +675,20 @@ __device__ __forceinline__ void bp_sep_footprint(const AgentDev &ag, const ViewC
     kstep = (int)sel3(cv[1], cv[2], cv[3], sa.W);
 }
 
+constexpr int kSepWaves = 8;  // waves along P2 per block (512 threads)
+
 template <class T, int RW, int PITCH, bool BPT>
-__global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda, BpSepArgs sa)
+__global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda,
+                                                          BpSepArgs sa)
 {
+    constexpr int NT = 64 * kSepWaves;
     // LDS holds (r[k][i], r[k][i+1] - r[k][i]) pairs so that the interior
     // update is S += r0 + du * d: one 64-bit LDS read and two FP32 ops.
     __shared__ float2 lds[2][RW * PITCH];
     const AgentDev &ag = ags[blockIdx.y];
     const int n1 = sel3(ag.X, ag.Y, ag.Z, sa.P1), n2 = sel3(ag.X, ag.Y, ag.Z, sa.P2), nw = sel3(ag.X, ag.Y, ag.Z, sa.W);
     const int s1 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P1), s2 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P2), sw = sel3(ag.Xs, ag.Ys, ag.Zs, sa.W);
-    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + 3) / 4, ntw = (nw + RW - 1) / RW;
+    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kSepWaves - 1) / kSepWaves, ntw = (nw + RW - 1) / RW;
     const int nb = nt1 * nt2 * ntw;
     // XCD-aware order: consecutive tiles (which share detector rows) run on
     // the same XCD's L2.
@@ -710,9 +697,10 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
     const int b1 = b % nt1, b2 = (b / nt1) % nt2, bw = b / (nt1 * nt2);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * 4, W0 = sw + bw * RW;
+    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kSepWaves, W0 = sw + bw * RW;
     const int p1 = P10 + lane, p2 = P20 + wid;
-    const int e1 = min(64, s1 + n1 - P10) - 1, e2 = min(4, s2 + n2 - P20) - 1, ew = min(RW, sw + nw - W0) - 1;
+    const int e1 = min(64, s1 + n1 - P10) - 1, e2 = min(kSepWaves, s2 + n2 - P20) - 1,
+              ew = min(RW, sw + nw - W0) - 1;
     const bool active = p1 < s1 + n1 && p2 < s2 + n2;
     const int nu = ag.nu, nv = ag.nv;
     const int64_t plane = (int64_t)nu * nv;
@@ -720,13 +708,21 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     const int X = sa.P1 == 0 ? p1 : (sa.P2 == 0 ? p2 : W0);
     const int Y = sa.P1 == 1 ? p1 : (sa.P2 == 1 ? p2 : W0);
     const int Z = sa.P1 == 2 ? p1 : (sa.P2 == 2 ? p2 : W0);
-    using St = BpSepStage<RW, PITCH>;
+    using St = BpSepStage<RW, PITCH, NT>;
 
     float S[RW];
 #pragma unroll
     for (int t = 0; t < RW; t++) S[t] = 0.f;
     float Cacc = 0.f;  // every row of the run is on the detector (host-checked)
     float2 pre[St::kPer];
+    // this thread's staging slots: fixed (row, column) of the footprint
+    int srow[St::kPer], scol[St::kPer];
+#pragma unroll
+    for (int q = 0; q < St::kPer; q++) {
+        const int e = threadIdx.x + NT * q;
+        srow[q] = e < St::kElems ? e / PITCH : RW;  // RW: no slot
+        scol[q] = e - (e / PITCH) * PITCH;
+    }
 
     auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
         const ViewCoef vc = ag.coef[a];
@@ -734,11 +730,9 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
         const uint32_t vbase = (uint32_t)(plane * a);
 #pragma unroll
         for (int q = 0; q < St::kPer; q++) {
-            const int e = threadIdx.x + 256 * q;
-            const int row = e / PITCH, col = e - row * PITCH;
-            const int gk = k0row + kstep * row, gi = fi0 + col;
+            const int gk = k0row + kstep * srow[q], gi = fi0 + scol[q];
             float v0 = 0.f, v1 = 0.f;
-            if (e < St::kElems && col < width && row <= ew && (unsigned)gk < (unsigned)nv) {
+            if (srow[q] <= ew && scol[q] < width) {
                 const uint32_t o = vbase + (uint32_t)gk * nu;
                 if ((unsigned)gi < (unsigned)nu) v0 = ldg<T>(src, o + gi);
                 if ((unsigned)(gi + 1) < (unsigned)nu) v1 = ldg<T>(src, o + gi + 1);
@@ -749,7 +743,7 @@ __global__ __launch_bounds__(256) void k_bp_sep(const AgentDev *__restrict__ ags
     auto commit = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < St::kPer; q++) {
-            const int e = threadIdx.x + 256 * q;
+            const int e = threadIdx.x + NT * q;
             if (e < St::kElems) lds[buf][e] = pre[q];
         }
     };
@@ -1201,7 +1195,7 @@ hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, i
     return launch_bp_t<float>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
 }
 
-int fp_stream_blocks(const AgentDev &ag) { return ((ag.nu + 3) / 4) * ((ag.nv + 63) / 64) * ag.A; }
+int fp_stream_blocks(const AgentDev &ag) { return (((ag.nu + 3) / 4) * ((ag.nv + 63) / 64) * ag.A + 7) / 8 * 8; }
 
 template <class T>
 static hipError_t launch_fp_stream_t(const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
@@ -1220,12 +1214,12 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
     return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, st);
 }
 
-constexpr int kSepPitch = 72;
+constexpr int kSepPitch = 80;
 
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 {
     auto ext = [&](int ax) { return ax == 0 ? ag.X : (ax == 1 ? ag.Y : ag.Z); };
-    const int nb = ((ext(P1) + 63) / 64) * ((ext(P2) + 3) / 4) * ((ext(W) + rw - 1) / rw);
+    const int nb = ((ext(P1) + 63) / 64) * ((ext(P2) + kSepWaves - 1) / kSepWaves) * ((ext(W) + rw - 1) / rw);
     return ((nb + 7) / 8) * 8;
 }
 
@@ -1235,9 +1229,9 @@ template <class T, int RW>
 static void launch_bp_sep_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, BpSepArgs sa, hipStream_t st)
 {
     if (bpt)
-        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, true>), grid, dim3(256), 0, st, d_ags, lambda, sa);
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, true>), grid, dim3(64 * kSepWaves), 0, st, d_ags, lambda, sa);
     else
-        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, false>), grid, dim3(256), 0, st, d_ags, lambda, sa);
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, false>), grid, dim3(64 * kSepWaves), 0, st, d_ags, lambda, sa);
 }
 
 template <class T>
