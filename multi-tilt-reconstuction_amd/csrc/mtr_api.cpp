@@ -148,7 +148,7 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc, const double *
     const int P1 = W == 0 ? 1 : 0, P2 = 3 - W - P1;
     const int lo[3] = {vol.Xstart, vol.Ystart, vol.Zstart}, hi[3] = {vol.Xend - 1, vol.Yend - 1, vol.Zend - 1};
     for (size_t v = 0; v < plan.size(); v++) {
-        const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * 7 + 5;
+        const double w = std::fabs(xc[4 * v + 1 + P1]) * 63 + std::fabs(xc[4 * v + 1 + P2]) * (bp_sep_waves() - 1) + 5;
         if (w > bp_sep_pitch()) return m;
         // every plane of the box must land on a detector row
         const double k0 = yc[4 * v] + yc[4 * v + 1 + W] * lo[W] - 1, k1 = yc[4 * v] + yc[4 * v + 1 + W] * hi[W] - 1;
@@ -159,6 +159,23 @@ BpMode bp_mode(const std::vector<FpPlan> &plan, const double *xc, const double *
     m.P1 = P1;
     m.P2 = P2;
     return m;
+}
+
+std::vector<BpSepView> bp_sep_views(const BpMode &m, const double *xc, const double *yc, int A)
+{
+    std::vector<BpSepView> out(m.sep ? A : 0);
+    const int waves = bp_sep_waves();
+    for (int v = 0; v < (int)out.size(); v++) {
+        BpSepView &b = out[v];
+        b.c0 = xc[4 * v] - 1.0;
+        b.a1 = xc[4 * v + 1 + m.P1];
+        b.a2 = xc[4 * v + 1 + m.P2];
+        b.umin = std::min(0.0, b.a1 * 63) + std::min(0.0, b.a2 * (waves - 1));
+        b.umax = std::max(0.0, b.a1 * 63) + std::max(0.0, b.a2 * (waves - 1));
+        b.k0 = (int)(yc[4 * v] - 1.0);
+        b.kstep = (int)yc[4 * v + 1 + m.W];
+    }
+    return out;
 }
 
 // Forward-projection streaming mode: every view row-separable along the
@@ -223,6 +240,7 @@ struct Agent {
     void *d_tmp = nullptr; // canonical copy of the working volume (fast != 0)
     ViewCoef *d_coef = nullptr;
     FpPlan *d_plan = nullptr;
+    BpSepView *d_bsv = nullptr;
     void *d_p = nullptr;
     bool p_owned = true;
     float *d_c = nullptr;
@@ -344,6 +362,7 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.Z = ctx->vol.Z;
     d.coef = ag.d_coef;
     d.plan = ag.d_plan;
+    d.bsv = ag.d_bsv;
     d.p = ag.d_p;
     d.c = ag.d_c;
     d.r = ag.d_r;
@@ -524,7 +543,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     Agent &ag = ctx->agents.back();
     auto undo = [&](int rc) {
         Agent &g = ctx->agents.back();
-        for (void *q : {(void *)g.d_coef, (void *)g.d_plan, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
+        for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
@@ -555,6 +574,13 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     } while (0)
     ALLOC(ag.d_coef, sizeof(ViewCoef) * A);
     ALLOC(ag.d_plan, sizeof(FpPlan) * A);
+    const std::vector<BpSepView> bsv = bp_sep_views(ag.bpm, xc, yc, A);
+    if (!bsv.empty()) {
+        ALLOC(ag.d_bsv, sizeof(BpSepView) * A);
+        if (hipMemcpyAsync(ag.d_bsv, bsv.data(), sizeof(BpSepView) * A, hipMemcpyHostToDevice, ctx->stream) !=
+            hipSuccess)
+            return undo(fail(MTR_ERR_HIP, "geometry upload failed"));
+    }
     ALLOC(ag.d_c, sizeof(float) * P);
     ALLOC(ag.d_r, es * P);
     ALLOC(ag.d_x, es * ctx->N);
@@ -608,7 +634,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
 
 // Temporary single-agent device state for the stateless operators (FP32).
 struct TmpAgent {
-    DevBuf coef, plan, x, s, c, r, tab;
+    DevBuf coef, plan, bsv, x, s, c, r, tab;
     AgentDev dev{};
     bool lanes_k = false;
     BpMode bpm;
@@ -646,6 +672,11 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     HIP_TRY(hipMalloc(&t.tab.p, sizeof(AgentDev)));
     HIP_TRY(hipMemcpyAsync(t.coef.p, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(t.plan.p, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream));
+    const std::vector<BpSepView> bsv = bp_sep_views(t.bpm, xc, yc, A);
+    if (!bsv.empty()) {
+        HIP_TRY(hipMalloc(&t.bsv.p, sizeof(BpSepView) * A));
+        HIP_TRY(hipMemcpyAsync(t.bsv.p, bsv.data(), sizeof(BpSepView) * A, hipMemcpyHostToDevice, ctx->stream));
+    }
     AgentDev &d = t.dev;
     d.nu = prj->X;
     d.nv = prj->Y;
@@ -661,6 +692,7 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     d.Z = vol->Z;
     d.coef = (const ViewCoef *)t.coef.p;
     d.plan = (const FpPlan *)t.plan.p;
+    d.bsv = (const BpSepView *)t.bsv.p;
     d.p = t.r.p;  // the stateless BP/BPT read their input from r
     d.c = (const float *)t.c.p;
     d.r = t.r.p;
@@ -775,7 +807,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ag : ctx->agents) {
-        for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
+        for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar})
             if (q) (void)hipFree(q);
     }

@@ -43,6 +43,15 @@ struct FpPlan {
     double m0, mi, mw, ml, Em;
 };
 
+// Per-view constants of the run-along-W back-projection (row-separable
+// views with u independent of W): u = c0 + a1*p1 + a2*p2 over the in-plane
+// axes (P1, P2); detector row of plane w = k0 + kstep*w; over a full tile
+// (64 x waves voxels) u spans [u(origin) + umin, u(origin) + umax].
+struct BpSepView {
+    double c0, a1, a2, umin, umax;
+    int k0, kstep;
+};
+
 // Device-side description of one agent, passed by value to kernels.
 struct AgentDev {
     int nu, nv, A;
@@ -51,6 +60,7 @@ struct AgentDev {
     int64_t sx, sy, sz;    // element strides of the working volume x
     const ViewCoef *coef;  // [A]
     const FpPlan *plan;    // [A]
+    const BpSepView *bsv;  // [A] (row-separable agents only)
     const void *p;         // projections [A][nv][nu] (storage precision)
     const float *c;        // FP normaliser [A][nv][nu] (FP32 always)
     void *r;               // residual [A][nv][nu]

@@ -16,6 +16,7 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
 // rw: voxels per thread along W (8, 16 or 32).
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw);
 int bp_sep_pitch();
+int bp_sep_waves();
 hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
                          int P1, int P2, int rw, hipStream_t st);
 

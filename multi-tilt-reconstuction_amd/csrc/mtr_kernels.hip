@@ -655,26 +655,6 @@ struct BpSepStage {
     static constexpr int kPer = (kElems + NT - 1) / NT;
 };
 
-template <class T, int RW, int PITCH>
-__device__ __forceinline__ void bp_sep_footprint(const AgentDev &ag, const ViewCoef &vc, const BpSepArgs &sa,
-                                                 int P10, int P20, int W0, int e1, int e2, int &fi0, int &width,
-                                                 int &k0row, int &kstep)
-{
-    const double *cu = vc.c, *cv = vc.c + 4;
-    const int X = sel3(P10, P20, W0, sa.P1 == 0 ? 0 : (sa.P2 == 0 ? 1 : 2));
-    const int Y = sel3(P10, P20, W0, sa.P1 == 1 ? 0 : (sa.P2 == 1 ? 1 : 2));
-    const int Z = sel3(P10, P20, W0, sa.P1 == 2 ? 0 : (sa.P2 == 2 ? 1 : 2));
-    const double u0 = coord(cu, X, Y, Z);
-    const double a1 = sel3(cu[1], cu[2], cu[3], sa.P1), a2 = sel3(cu[1], cu[2], cu[3], sa.P2);
-    const double ulo = u0 + fmin(0.0, a1 * e1) + fmin(0.0, a2 * e2);
-    const double uhi = u0 + fmax(0.0, a1 * e1) + fmax(0.0, a2 * e2);
-    fi0 = max((int)floor(ulo) - 1, -1);
-    const int fi1 = min((int)floor(uhi) + 2, ag.nu);
-    width = fi1 - fi0 + 1;
-    k0row = (int)coord(cv, X, Y, Z);  // v is integral; row of W0
-    kstep = (int)sel3(cv[1], cv[2], cv[3], sa.W);
-}
-
 constexpr int kSepWaves = 8;  // waves along P2 per block (512 threads)
 
 template <class T, int RW, int PITCH, bool BPT>
@@ -699,8 +679,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kSepWaves, W0 = sw + bw * RW;
     const int p1 = P10 + lane, p2 = P20 + wid;
-    const int e1 = min(64, s1 + n1 - P10) - 1, e2 = min(kSepWaves, s2 + n2 - P20) - 1,
-              ew = min(RW, sw + nw - W0) - 1;
+    const int ew = min(RW, sw + nw - W0) - 1;
     const bool active = p1 < s1 + n1 && p2 < s2 + n2;
     const int nu = ag.nu, nv = ag.nv;
     const int64_t plane = (int64_t)nu * nv;
@@ -725,19 +704,25 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     }
 
     auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
-        const ViewCoef vc = ag.coef[a];
-        bp_sep_footprint<T, RW, PITCH>(ag, vc, sa, P10, P20, W0, e1, e2, fi0, width, k0row, kstep);
+        const BpSepView bv = ag.bsv[a];
+        const double u0 = fma(bv.a1, (double)P10, fma(bv.a2, (double)P20, bv.c0));
+        fi0 = max((int)floor(u0 + bv.umin) - 1, -1);
+        const int fi1 = min((int)floor(u0 + bv.umax) + 2, nu);
+        width = fi1 - fi0 + 1;
+        k0row = bv.k0 + bv.kstep * W0;
+        kstep = bv.kstep;
         const uint32_t vbase = (uint32_t)(plane * a);
 #pragma unroll
         for (int q = 0; q < St::kPer; q++) {
+            // branch-free: clamp the address into the detector, select 0 outside
             const int gk = k0row + kstep * srow[q], gi = fi0 + scol[q];
-            float v0 = 0.f, v1 = 0.f;
-            if (srow[q] <= ew && scol[q] < width) {
-                const uint32_t o = vbase + (uint32_t)gk * nu;
-                if ((unsigned)gi < (unsigned)nu) v0 = ldg<T>(src, o + gi);
-                if ((unsigned)(gi + 1) < (unsigned)nu) v1 = ldg<T>(src, o + gi + 1);
-            }
-            pre[q] = make_float2(v0, v1 - v0);
+            const bool ok = srow[q] <= ew && scol[q] < width;
+            const bool ok0 = ok && (unsigned)gi < (unsigned)nu, ok1 = ok && (unsigned)(gi + 1) < (unsigned)nu;
+            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * nu;
+            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1));
+            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1));
+            const float r0 = ok0 ? v0 : 0.f, r1 = ok1 ? v1 : 0.f;
+            pre[q] = make_float2(r0, r1 - r0);
         }
     };
     auto commit = [&](int buf) {
@@ -755,8 +740,8 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     for (int a = 0; a < ag.A; a++) {
         const int cfi0 = fi0, cwidth = width;
         if (a + 1 < ag.A) fetch(a + 1, fi0, width, k0row, kstep);
-        const ViewCoef vc = ag.coef[a];
-        const double u = coord(vc.c, X, Y, Z);
+        const BpSepView bv = ag.bsv[a];
+        const double u = fma(bv.a1, (double)p1, fma(bv.a2, (double)p2, bv.c0));
         const double fu = floor(u);
         const int i0 = (int)fu;
         const float du = (float)(u - fu);
@@ -1224,6 +1209,7 @@ int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 }
 
 int bp_sep_pitch() { return kSepPitch; }
+int bp_sep_waves() { return kSepWaves; }
 
 template <class T, int RW>
 static void launch_bp_sep_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, BpSepArgs sa, hipStream_t st)
