@@ -371,15 +371,18 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     __shared__ int2 taps[4][72];  // per wave: (offset, weight bits), 64 + 8 padding
     const AgentDev &ag = ags[blockIdx.y];
     const int nti = (ag.nu + 3) / 4, ntk = (ag.nv + 63) / 64;
-    // XCD-aware order: adjacent ray groups of one view share an XCD's L2
+    // XCD-aware order: each XCD takes a contiguous range of (plane group,
+    // column group, view) with the view fastest, so the rays in flight on
+    // one XCD are all views of a few adjacent columns of one plane group and
+    // the voxels they share stay in that XCD's L2 across views.
     const int nbt = nti * ntk * ag.A, per = (nbt + 7) / 8;
     if (blockIdx.x >= (unsigned)(per * 8)) return;
     int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (b >= nbt) return;
+    const int a = b % ag.A;
+    b /= ag.A;
     const int bi = b % nti;
-    b /= nti;
-    const int bk = b % ntk;
-    const int a = b / ntk;
+    const int bk = b / nti;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int i = bi * 4 + wid;
     const int k = bk * 64 + lane;
@@ -664,7 +667,9 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     constexpr int NT = 64 * kSepWaves;
     // LDS holds (r[k][i], r[k][i+1] - r[k][i]) pairs so that the interior
     // update is S += r0 + du * d: one 64-bit LDS read and two FP32 ops.
-    __shared__ float2 lds[2][RW * PITCH];
+    constexpr int kStage = 2 * RW * PITCH, kEpi = kSepWaves * 64 * 18 / 2;
+    __shared__ float2 lds_raw[kStage > kEpi ? kStage : kEpi];
+    float2(*lds)[RW * PITCH] = reinterpret_cast<float2(*)[RW * PITCH]>(lds_raw);
     const AgentDev &ag = ags[blockIdx.y];
     const int n1 = sel3(ag.X, ag.Y, ag.Z, sa.P1), n2 = sel3(ag.X, ag.Y, ag.Z, sa.P2), nw = sel3(ag.X, ag.Y, ag.Z, sa.W);
     const int s1 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P1), s2 = sel3(ag.Xs, ag.Ys, ag.Zs, sa.P2), sw = sel3(ag.Xs, ag.Ys, ag.Zs, sa.W);
@@ -680,7 +685,6 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kSepWaves, W0 = sw + bw * RW;
     const int p1 = P10 + lane, p2 = P20 + wid;
     const int ew = min(RW, sw + nw - W0) - 1;
-    const bool active = p1 < s1 + n1 && p2 < s2 + n2;
     const int nu = ag.nu, nv = ag.nv;
     const int64_t plane = (int64_t)nu * nv;
     const auto src = gview(reinterpret_cast<const T *>(BPT ? ag.p : ag.r));
@@ -770,19 +774,46 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         if (a + 1 < ag.A) commit((a + 1) & 1);
         __syncthreads();
     }
-    if (!active) return;
+    // Epilogue: each lane holds a run along W (contiguous in memory) at its
+    // own P1.  Stored directly, every store instruction would touch 64
+    // cache lines 4 bytes at a time; instead the wave transposes its runs
+    // through LDS, CH voxels of each run at a time, so that CH lanes write
+    // CH consecutive voxels of one run.  Same arithmetic as per-lane.
+    constexpr int CH = RW < 16 ? RW : 16, LPI = 64 / CH;  // chunk, lines per instruction
+    static_assert(RW % CH == 0 && 64 % CH == 0, "run length must divide into chunks");
+    float *ep = reinterpret_cast<float *>(lds_raw) + wid * 64 * 18;  // 64 x 17 values + 64 C
+    float *cl = ep + 64 * 17;
     T *x = reinterpret_cast<T *>(ag.x);
-    const int64_t sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, sa.W);
-    const int64_t j0 = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy + (int64_t)(Z - ag.Zs) * ag.sz;
+    const int64_t sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, sa.W), sP1 = sel3<int64_t>(ag.sx, ag.sy, ag.sz, sa.P1);
+    // voxel index of line 0 of this wave (P1 = P10)
+    const int64_t jb = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy + (int64_t)(Z - ag.Zs) * ag.sz -
+                       (int64_t)lane * sP1;
+    const bool wave_ok = p2 < s2 + n2;
+    cl[lane] = Cacc;
 #pragma unroll
-    for (int t = 0; t < RW; t++) {
-        if (t > ew) break;
-        const int64_t j = j0 + t * sW;
-        if (BPT) {
-            stf<T>(x, j, Cacc != 0.0f ? S[t] / Cacc : 0.0f);
-        } else if (Cacc != 0.0f) {
-            stf<T>(x, j, fmaf(S[t] / Cacc, lambda, ldf<T>(x, j)));
+    for (int h = 0; h < RW / CH; h++) {
+#pragma unroll
+        for (int t = 0; t < CH; t++) ep[lane * 17 + t] = S[CH * h + t];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int tt = lane % CH, tw = CH * h + tt;
+#pragma unroll 4
+        for (int i = 0; i < CH; i++) {
+            const int line = i * LPI + lane / CH;
+            const float sv = ep[line * 17 + tt], cv = cl[line];
+            if (wave_ok && tw <= ew && P10 + line < s1 + n1) {
+                const int64_t j = jb + line * sP1 + tw * sW;
+                if (BPT) {
+                    stf<T>(x, j, cv != 0.0f ? sv / cv : 0.0f);
+                } else if (cv != 0.0f) {
+                    stf<T>(x, j, fmaf(sv / cv, lambda, ldf<T>(x, j)));
+                }
+            }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
