@@ -241,8 +241,13 @@ struct Agent {
     ViewCoef *d_coef = nullptr;
     FpPlan *d_plan = nullptr;
     BpSepView *d_bsv = nullptr;
+    // Row-separable agents on the streaming FP and separable BP keep p, c
+    // and r row-fastest ([A][nu][nv]): the 64 lanes of both kernels run
+    // along detector rows.
+    bool rowfast = false;
     void *d_p = nullptr;
     bool p_owned = true;
+    float *d_p_std = nullptr;  // [A][nv][nu] copy of p for the plain form (rowfast only)
     float *d_c = nullptr;
     void *d_r = nullptr;
     void *d_x = nullptr;
@@ -363,6 +368,8 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.coef = ag.d_coef;
     d.plan = ag.d_plan;
     d.bsv = ag.d_bsv;
+    d.pk = ag.rowfast ? 1 : ag.prj.X;
+    d.pi = ag.rowfast ? ag.prj.Y : 1;
     d.p = ag.d_p;
     d.c = ag.d_c;
     d.r = ag.d_r;
@@ -472,6 +479,24 @@ int ensure_plain_scratch(mtr_ctx *ctx, int64_t P)
     return MTR_OK;
 }
 
+// Projections in the [A][nv][nu] layout the plain form keeps (FP32).
+int plain_p(mtr_ctx *ctx, Agent &ag, const float **out)
+{
+    if (!ag.rowfast) {
+        *out = (const float *)ag.d_p;
+        return MTR_OK;
+    }
+    if (!ag.d_p_std) {
+        HIP_TRY(hipMalloc(&ag.d_p_std, sizeof(float) * ag.P()));
+        const int64_t ss[3] = {ag.prj.Y, 1, (int64_t)ag.prj.X * ag.prj.Y};
+        const int64_t ds[3] = {1, ag.prj.X, (int64_t)ag.prj.X * ag.prj.Y};
+        HIP_TRY(launch_permute(MTR_PREC_FP32, ag.d_p, ag.d_p_std, ag.prj.X, ag.prj.Y, ag.prj.AngN, ss, ds,
+                               ctx->stream));
+    }
+    *out = ag.d_p_std;
+    return MTR_OK;
+}
+
 // Plain form: one agent, splat FP with atomics into zeroed s/c (c
 // recomputed every iteration), separate residual kernel, direct-gather BP,
 // host synchronisation after each iteration.
@@ -481,6 +506,8 @@ int sirt_plain(mtr_ctx *ctx, int a, int iters, float lambda)
     if (ctx->precision != MTR_PREC_FP32)
         return fail(MTR_ERR_STATE, "plain form runs FP32 storage only");
     MTR_TRY(ensure_plain_scratch(ctx, ag.P()));
+    const float *pstd = nullptr;
+    MTR_TRY(plain_p(ctx, ag, &pstd));
     const AgentDev d = make_dev(ctx, ag);
     for (int it = 0; it < iters; it++) {
         HIP_TRY(hipMemsetAsync(ctx->d_s, 0, sizeof(float) * ag.P(), ctx->stream));
@@ -491,7 +518,7 @@ int sirt_plain(mtr_ctx *ctx, int a, int iters, float lambda)
         }
         {
             Timed t(ctx, MTR_K_ELEMENTWISE);
-            HIP_TRY(launch_residual(ctx->precision, ag.P(), ag.d_p, ctx->d_s, ctx->d_cc, ag.d_r, ctx->stream));
+            HIP_TRY(launch_residual(ctx->precision, ag.P(), pstd, ctx->d_s, ctx->d_cc, ag.d_r, ctx->stream));
         }
         {
             Timed t(ctx, MTR_K_BP);
@@ -544,7 +571,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     auto undo = [&](int rc) {
         Agent &g = ctx->agents.back();
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
-                        g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar})
+                        g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -563,6 +590,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     ag.fast = ag.fp_nc ? sW : 0;
     layout_strides(ag.fast, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st);
+    ag.rowfast = ag.fp_nc > 0 && ag.bpm.sep;
     const size_t es = elem_size(ctx->precision);
     const int64_t P = ag.P();
 #define ALLOC(ptr, bytes)                                                                 \
@@ -594,30 +622,55 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         hipMemcpyAsync(ag.d_plan, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess)
         return undo(fail(MTR_ERR_HIP, "geometry upload failed"));
-    if (ctx->precision == MTR_PREC_FP32 && p_dev) {
+    if (ctx->precision == MTR_PREC_FP32 && p_dev && !ag.rowfast) {
         ag.d_p = const_cast<float *>(p_dev);
         ag.p_owned = false;
     } else {
         ALLOC(ag.d_p, es * P);
         ag.p_owned = true;
-        if (p_host && ctx->precision == MTR_PREC_FP32) {
+        // p arrives FP32 [A][nv][nu]: convert to storage precision and, for
+        // row-fastest agents, transpose each view; both through a staging
+        // buffer where needed
+        const bool direct = ctx->precision == MTR_PREC_FP32 && !ag.rowfast;
+        if (p_host && direct) {
             if (hipMemcpyAsync(ag.d_p, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
                 return undo(fail(MTR_ERR_HIP, "projection upload failed"));
         } else {
             const float *src = p_dev;
             float *tmp = nullptr;
+            void *conv = nullptr;
+            auto cleanup = [&]() {
+                (void)hipStreamSynchronize(ctx->stream);
+                if (tmp) (void)hipFree(tmp);
+                if (conv) (void)hipFree(conv);
+            };
             if (p_host) {
                 if (hipMalloc(&tmp, sizeof(float) * P) != hipSuccess)
                     return undo(fail(MTR_ERR_OOM, "staging buffer"));
                 if (hipMemcpyAsync(tmp, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
-                    (void)hipFree(tmp);
+                    cleanup();
                     return undo(fail(MTR_ERR_HIP, "projection upload failed"));
                 }
                 src = tmp;
             }
-            hipError_t e = launch_to_storage(ctx->precision, P, src, ag.d_p, ctx->stream);
-            (void)hipStreamSynchronize(ctx->stream);
-            if (tmp) (void)hipFree(tmp);
+            hipError_t e = hipSuccess;
+            if (ag.rowfast) {
+                if (ctx->precision != MTR_PREC_FP32) {
+                    if (hipMalloc(&conv, es * P) != hipSuccess) {
+                        cleanup();
+                        return undo(fail(MTR_ERR_OOM, "staging buffer"));
+                    }
+                    e = launch_to_storage(ctx->precision, P, src, conv, ctx->stream);
+                }
+                const int64_t ss[3] = {1, prj->X, (int64_t)prj->X * prj->Y};
+                const int64_t ds[3] = {prj->Y, 1, (int64_t)prj->X * prj->Y};
+                if (e == hipSuccess)
+                    e = launch_permute(ctx->precision, conv ? conv : (const void *)src, ag.d_p, prj->X, prj->Y,
+                                       prj->AngN, ss, ds, ctx->stream);
+            } else {
+                e = launch_to_storage(ctx->precision, P, src, ag.d_p, ctx->stream);
+            }
+            cleanup();
             if (e != hipSuccess) return undo(fail(MTR_ERR_HIP, "projection conversion failed"));
         }
     }
@@ -693,6 +746,8 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     d.coef = (const ViewCoef *)t.coef.p;
     d.plan = (const FpPlan *)t.plan.p;
     d.bsv = (const BpSepView *)t.bsv.p;
+    d.pk = prj->X;
+    d.pi = 1;
     d.p = t.r.p;  // the stateless BP/BPT read their input from r
     d.c = (const float *)t.c.p;
     d.r = t.r.p;
@@ -808,7 +863,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ag : ctx->agents) {
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
-                        ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar})
+                        ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
@@ -991,9 +1046,10 @@ int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form
         if (ctx->precision != MTR_PREC_FP32) return fail(MTR_ERR_STATE, "plain form runs FP32 storage only");
         // BPT per agent, sum, then per step: z, SIRT, Mann per agent, mean.
         for (int a = 0; a < K; a++) {
+            const float *pstd = nullptr;
+            MTR_TRY(plain_p(ctx, ctx->agents[a], &pstd));
             Timed t(ctx, MTR_K_BPT);
-            HIP_TRY(launch_plain_bp(make_dev(ctx, ctx->agents[a]), (const float *)ctx->agents[a].d_p, 1.0f, 1,
-                                    ctx->stream));
+            HIP_TRY(launch_plain_bp(make_dev(ctx, ctx->agents[a]), pstd, 1.0f, 1, ctx->stream));
         }
         MTR_TRY(init_from_bpt(ctx));
         HIP_TRY(hipStreamSynchronize(ctx->stream));

@@ -57,13 +57,17 @@ struct AgentDev {
     int nu, nv, A;
     int Xs, Xe, Ys, Ye, Zs, Ze;  // volume box (absolute voxel coordinates)
     int X, Y, Z;
+    // Element strides of detector row k and column i within one view of
+    // p, c and r: (nu, 1) for the [A][nv][nu] layout, (1, nv) for the
+    // row-fastest [A][nu][nv] layout of row-separable agents.
+    int pk, pi;
     int64_t sx, sy, sz;    // element strides of the working volume x
     const ViewCoef *coef;  // [A]
     const FpPlan *plan;    // [A]
     const BpSepView *bsv;  // [A] (row-separable agents only)
-    const void *p;         // projections [A][nv][nu] (storage precision)
-    const float *c;        // FP normaliser [A][nv][nu] (FP32 always)
-    void *r;               // residual [A][nv][nu]
+    const void *p;         // projections (storage precision), strides pk/pi
+    const float *c;        // FP normaliser (FP32 always), strides pk/pi
+    void *r;               // residual, strides pk/pi
     void *x;               // working volume, strides sx/sy/sz
     double *partial;       // per-block FP64 partial sums
 };

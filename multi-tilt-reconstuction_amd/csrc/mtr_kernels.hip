@@ -48,6 +48,16 @@ __device__ __forceinline__ void stf<__hip_bfloat16>(__hip_bfloat16 *p, int64_t i
 template <>
 __device__ __forceinline__ void stf<__half>(__half *p, int64_t i, float v) { p[i] = __float2half(v); }
 
+// Value of v after a round trip through storage type T.
+template <class T>
+__device__ __forceinline__ float rnd(float v);
+template <>
+__device__ __forceinline__ float rnd<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float rnd<__hip_bfloat16>(float v) { return __bfloat162float(__float2bfloat16(v)); }
+template <>
+__device__ __forceinline__ float rnd<__half>(float v) { return __half2float(__float2half(v)); }
+
 // Global (address space 1) view of a pointer held in a struct: lets the
 // compiler emit global loads with a scalar base instead of flat loads.
 template <class T>
@@ -329,7 +339,7 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
         else
             acc = fp_ray_general<T, MODE == 2>(ag, vc, pl, i, k);
     }
-    const int64_t n = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.nu + i;
+    const int64_t n = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)i * ag.pi;
     if (MODE == 0) {
         double r2 = 0.0;
         if (active) {
@@ -338,7 +348,7 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
             if (cc != 0.0f) sv = sv / cc;
             const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), n) - sv;
             stf<T>(reinterpret_cast<T *>(ag.r), n, rv);
-            const double rs = (double)ldf<T>(reinterpret_cast<const T *>(ag.r), n);
+            const double rs = (double)rnd<T>(rv);  // the stored value
             r2 = rs * rs;
         }
         __shared__ double sh[4];
@@ -467,13 +477,15 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     }
     double r2 = 0.0;
     if (lane_ok) {
-        const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.nu + i;
+        // with the row-fastest layout (pk = 1) the 64 lanes touch 64
+        // consecutive elements of p, c and r
+        const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)i * ag.pi;
         const float cc = ag.c[nidx];
         float sv = acc;
         if (cc != 0.0f) sv = sv / cc;
         const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), nidx) - sv;
         stf<T>(reinterpret_cast<T *>(ag.r), nidx, rv);
-        const double rs = (double)ldf<T>(reinterpret_cast<const T *>(ag.r), nidx);
+        const double rs = (double)rnd<T>(rv);  // the stored value
         r2 = rs * rs;
     }
     __shared__ double sh[4];
@@ -698,13 +710,18 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     for (int t = 0; t < RW; t++) S[t] = 0.f;
     float Cacc = 0.f;  // every row of the run is on the detector (host-checked)
     float2 pre[St::kPer];
-    // this thread's staging slots: fixed (row, column) of the footprint
+    // this thread's staging slots: fixed (row, column) of the footprint.
+    // Consecutive threads take consecutive columns in the [nv][nu] layout
+    // and consecutive rows in the row-fastest [nu][nv] layout, so the
+    // staging loads are contiguous either way.
+    const bool rowfast = ag.pk == 1;
     int srow[St::kPer], scol[St::kPer];
 #pragma unroll
     for (int q = 0; q < St::kPer; q++) {
         const int e = threadIdx.x + NT * q;
-        srow[q] = e < St::kElems ? e / PITCH : RW;  // RW: no slot
-        scol[q] = e - (e / PITCH) * PITCH;
+        const int r = rowfast ? e % RW : e / PITCH, cidx = rowfast ? e / RW : e % PITCH;
+        srow[q] = e < St::kElems ? r : RW;  // RW: no slot
+        scol[q] = cidx;
     }
 
     auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
@@ -722,19 +739,17 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
             const int gk = k0row + kstep * srow[q], gi = fi0 + scol[q];
             const bool ok = srow[q] <= ew && scol[q] < width;
             const bool ok0 = ok && (unsigned)gi < (unsigned)nu, ok1 = ok && (unsigned)(gi + 1) < (unsigned)nu;
-            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * nu;
-            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1));
-            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1));
+            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * (uint32_t)ag.pk;
+            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1) * (uint32_t)ag.pi);
+            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1) * (uint32_t)ag.pi);
             const float r0 = ok0 ? v0 : 0.f, r1 = ok1 ? v1 : 0.f;
             pre[q] = make_float2(r0, r1 - r0);
         }
     };
     auto commit = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < St::kPer; q++) {
-            const int e = threadIdx.x + NT * q;
-            if (e < St::kElems) lds[buf][e] = pre[q];
-        }
+        for (int q = 0; q < St::kPer; q++)
+            if (srow[q] < RW) lds[buf][srow[q] * PITCH + scol[q]] = pre[q];
     };
 
     int fi0, width, k0row, kstep;
@@ -1230,7 +1245,9 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
     return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, st);
 }
 
-constexpr int kSepPitch = 80;
+// odd (in float2 units) so that a column of 32 rows written by 32 lanes
+// spreads over all LDS banks
+constexpr int kSepPitch = 81;
 
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 {
