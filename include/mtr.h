@@ -124,8 +124,27 @@ int mtr_agent_add(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc
 int mtr_agent_add_device(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc,
                          const double *yc, const float *p_dev, int *agent_id);
 int mtr_agent_count(const mtr_ctx *ctx, int *count);
-/* Bytes of HBM an agent of this shape occupies at the context precision. */
+/* Upper bound of the HBM bytes an agent of this shape occupies at the
+ * context precision (whatever kernels its geometry selects). */
 int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size_t *bytes);
+/* Kernels the geometry of an agent selected (no counterpart in the
+ * reference, which has one kernel per operator):
+ *   forward projection  MTR_FPK_GENERAL  ray-driven gather over the planes
+ *                                        of the driving axis
+ *                       MTR_FPK_STREAM   row-separable views: one ray per
+ *                                        wave over 64 adjacent planes
+ *                       MTR_FPK_COLUMN   v fixed per (X, Y) column in every
+ *                                        view: strip accumulation
+ *   back-projection     MTR_BPK_GENERAL, MTR_BPK_SEPARABLE (runs along the
+ *                                        plane axis), MTR_BPK_COLUMN (runs
+ *                                        along Z) */
+#define MTR_FPK_GENERAL 0
+#define MTR_FPK_STREAM 1
+#define MTR_FPK_COLUMN 2
+#define MTR_BPK_GENERAL 0
+#define MTR_BPK_SEPARABLE 1
+#define MTR_BPK_COLUMN 2
+int mtr_agent_kernels(const mtr_ctx *ctx, int agent, int *fp_kind, int *bp_kind);
 
 /* ---- the reference's hot-path entry points, device-resident ---------- */
 /* BPT of one agent into its working volume (zeroed first).  out_host

@@ -36,6 +36,8 @@ from ._lib import (  # noqa: F401
 )
 
 PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
+FPK_GENERAL, FPK_STREAM, FPK_COLUMN = 0, 1, 2
+BPK_GENERAL, BPK_SEPARABLE, BPK_COLUMN = 0, 1, 2
 FORM_FAST, FORM_PLAIN = 0, 1
 VOL_WORK, VOL_W, VOL_WBAR = 0, 1, 2
 K_FP, K_BP, K_BPT, K_CONSENSUS, K_NORMALISER, K_ELEMENTWISE, K_SPLAT = range(7)
@@ -145,6 +147,12 @@ class Context:
         b = _ct.c_size_t()
         _check(lib.mtr_agent_footprint(self._h, _ct.byref(prj), _ct.byref(b)), "agent_footprint")
         return b.value
+
+    def agent_kernels(self, agent: int) -> tuple[int, int]:
+        """(forward, back) projection kernels chosen for an agent: FPK_* and BPK_*."""
+        f, b = _ct.c_int(), _ct.c_int()
+        _check(lib.mtr_agent_kernels(self._h, agent, _ct.byref(f), _ct.byref(b)), "agent_kernels")
+        return f.value, b.value
 
     # -- hot path ----------------------------------------------------------
     def bpt(self, agent: int, out: bool = True):

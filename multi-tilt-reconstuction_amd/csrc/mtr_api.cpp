@@ -197,6 +197,72 @@ int fp_stream_nc(const std::vector<FpPlan> &plan, const double *xc, int *W_out)
     return nc;
 }
 
+// Column plan: every view has v independent of Z (yc3 = 0) and the same
+// v coefficients, so each (X, Y) column keeps its detector rows k0, k0 + 1
+// and weights in all views.  The back-projection tile (64 x 8 columns, runs
+// of rw along Z) must stage its rows times its u range within the LDS
+// budget; lanes go along the axis that gives the smaller footprint.
+struct ColMode {
+    bool on = false;
+    int p1y = 0;    // lanes along Y
+    int pitch = 0;  // staged pairs per row (odd)
+    int rw = 32;
+};
+
+ColMode col_mode(const double *xc, const double *yc, int A, const mtr_projection_size &prj, int rw)
+{
+    ColMode m;
+    if (prj.X > fp_col_max_nu()) return m;
+    for (int v = 0; v < A; v++)
+        for (int q = 0; q < 4; q++)
+            if (yc[4 * v + q] != yc[q]) return m;
+    if (yc[3] != 0.0 || (yc[1] == 0.0 && yc[2] == 0.0)) return m;
+    const int waves = bp_col_waves();
+    int best = -1;
+    for (int p1y = 0; p1y < 2; p1y++) {
+        const double r1 = std::fabs(yc[1 + p1y]), r2 = std::fabs(yc[2 - p1y]);
+        const int rows = (int)std::floor(63 * r1 + (waves - 1) * r2) + 3;
+        double wmax = 0.0;
+        for (int v = 0; v < A; v++) {
+            const double w = 63 * std::fabs(xc[4 * v + 1 + p1y]) + (waves - 1) * std::fabs(xc[4 * v + 2 - p1y]) +
+                             (rw - 1) * std::fabs(xc[4 * v + 3]);
+            wmax = std::max(wmax, w);
+        }
+        int pitch = (int)std::floor(wmax) + 4;
+        if (pitch % 2 == 0) pitch++;
+        const int elems = rows * pitch;
+        if (elems <= col_elems() && (best < 0 || elems < best)) {
+            best = elems;
+            m.on = true;
+            m.p1y = p1y;
+            m.pitch = pitch;
+            m.rw = rw;
+        }
+    }
+    return m;
+}
+
+// Per-view u extremes of a column-plan back-projection tile relative to
+// its origin (the tile spans 64 lanes along P1, the waves along P2 and rw
+// planes along Z).
+std::vector<BpSepView> bp_col_views(const ColMode &m, const double *xc, int A)
+{
+    std::vector<BpSepView> out(A);
+    const int waves = bp_col_waves();
+    for (int v = 0; v < A; v++) {
+        BpSepView &b = out[v];
+        std::memset(&b, 0, sizeof b);
+        const double e[3] = {m.p1y ? (double)(waves - 1) : 63.0, m.p1y ? 63.0 : (double)(waves - 1), (double)(m.rw - 1)};
+        for (int q = 0; q < 3; q++) {
+            const double d = xc[4 * v + 1 + q] * e[q];
+            b.umin += std::min(0.0, d);
+            b.umax += std::max(0.0, d);
+        }
+        b.c0 = xc[4 * v];
+    }
+    return out;
+}
+
 // Element strides of a volume whose contiguous axis is `fast` (-1 or 0:
 // the reference's [Z][Y][X]); the other two axes keep X-before-Y-before-Z
 // order.
@@ -248,6 +314,11 @@ struct Agent {
     void *d_p = nullptr;
     bool p_owned = true;
     float *d_p_std = nullptr;  // [A][nv][nu] copy of p for the plain form (rowfast only)
+    ColMode col;               // column plan (oblique in-plane axes)
+    int2 *d_col_xy = nullptr;
+    float2 *d_col_w = nullptr;
+    int *d_strip = nullptr;
+    float *d_pm = nullptr, *d_pn = nullptr;
     float *d_c = nullptr;
     void *d_r = nullptr;
     void *d_x = nullptr;
@@ -378,6 +449,14 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.sx = ag.st[0];
     d.sy = ag.st[1];
     d.sz = ag.st[2];
+    d.col_xy = ag.d_col_xy;
+    d.col_w = ag.d_col_w;
+    d.strip = ag.d_strip;
+    d.nstrip = ag.col.on ? ag.prj.Y + 1 : 0;
+    d.col_p1y = ag.col.p1y;
+    d.col_pitch = ag.col.pitch;
+    d.pm = ag.d_pm;
+    d.pn = ag.d_pn;
     return d;
 }
 
@@ -407,8 +486,15 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
     if (ag.c_ready) return MTR_OK;
     {
         Timed t(ctx, MTR_K_NORMALISER);
-        HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + a, 1, ag.nfp, 2, ag.lanes_k, nullptr, ag.d_c,
-                          ctx->stream));
+        if (ag.col.on) {
+            const AgentDev d = make_dev(ctx, ag);
+            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), ag.prj.X, true, ctx->stream));
+            HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + a, 1, fp_col_res_blocks(d), 2, ag.d_c,
+                                      ctx->stream));
+        } else {
+            HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + a, 1, ag.nfp, 2, ag.lanes_k, nullptr, ag.d_c,
+                              ctx->stream));
+        }
     }
     ag.c_ready = true;
     return MTR_OK;
@@ -423,8 +509,25 @@ int fp_group(mtr_ctx *ctx, const std::vector<int> &ids)
         const Agent &a0 = ctx->agents[ids[s]];
         size_t e = s + 1;
         while (e < ids.size() && ids[e] == ids[e - 1] + 1 && ctx->agents[ids[e]].lanes_k == a0.lanes_k &&
-               ctx->agents[ids[e]].fp_nc == a0.fp_nc)
+               ctx->agents[ids[e]].fp_nc == a0.fp_nc && ctx->agents[ids[e]].col.on == a0.col.on)
             e++;
+        if (a0.col.on) {
+            int mb = 0, mr = 0, nu = 0;
+            for (size_t q = s; q < e; q++) {
+                Agent &ag = ctx->agents[ids[q]];
+                const AgentDev d = make_dev(ctx, ag);
+                ag.npartial = fp_col_res_blocks(d);
+                mb = std::max(mb, fp_col_blocks(d));
+                mr = std::max(mr, ag.npartial);
+                nu = std::max(nu, ag.prj.X);
+            }
+            Timed t(ctx, MTR_K_FP);
+            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, nu, false, ctx->stream));
+            HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mr, 0, nullptr,
+                                      ctx->stream));
+            s = e;
+            continue;
+        }
         int mb = 0;
         for (size_t q = s; q < e; q++) {
             Agent &ag = ctx->agents[ids[q]];
@@ -451,10 +554,17 @@ int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
     while (s < end) {
         const BpMode &m = ctx->agents[s].bpm;
         int e = s + 1;
-        while (e < end && ctx->agents[e].bpm.sep == m.sep && ctx->agents[e].bpm.W == m.W) e++;
+        const ColMode &cm = ctx->agents[s].col;
+        while (e < end && ctx->agents[e].bpm.sep == m.sep && ctx->agents[e].bpm.W == m.W &&
+               ctx->agents[e].col.on == cm.on && ctx->agents[e].col.rw == cm.rw)
+            e++;
         const AgentDev d = make_dev(ctx, ctx->agents[s]);
         Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
-        if (m.sep) {
+        if (cm.on) {
+            int mb = 0;
+            for (int q = s; q < e; q++) mb = std::max(mb, bp_col_blocks(make_dev(ctx, ctx->agents[q]), cm.rw));
+            HIP_TRY(launch_bp_col(ctx->precision, ctx->d_ags + s, e - s, mb, bpt, lambda, cm.rw, ctx->stream));
+        } else if (m.sep) {
             HIP_TRY(launch_bp_sep(ctx->precision, ctx->d_ags + s, e - s, bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw), bpt,
                                   lambda, m.W, m.P1, m.P2, m.rw, ctx->stream));
         } else {
@@ -571,7 +681,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     auto undo = [&](int rc) {
         Agent &g = ctx->agents.back();
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
-                        g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std})
+                        g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
+                        (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -589,6 +700,12 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     int sW = 0;
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     ag.fast = ag.fp_nc ? sW : 0;
+    if (!ag.fp_nc && !ag.bpm.sep) {
+        int rw = 32;
+        if (const char *e = std::getenv("MTR_COL_RW")) rw = std::atoi(e) == 16 ? 16 : 32;
+        ag.col = col_mode(xc, yc, A, *prj, rw);
+        if (ag.col.on) ag.fast = 2;  // Z-contiguous columns
+    }
     layout_strides(ag.fast, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st);
     ag.rowfast = ag.fp_nc > 0 && ag.bpm.sep;
     const size_t es = elem_size(ctx->precision);
@@ -602,7 +719,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     } while (0)
     ALLOC(ag.d_coef, sizeof(ViewCoef) * A);
     ALLOC(ag.d_plan, sizeof(FpPlan) * A);
-    const std::vector<BpSepView> bsv = bp_sep_views(ag.bpm, xc, yc, A);
+    const std::vector<BpSepView> bsv = ag.col.on ? bp_col_views(ag.col, xc, A) : bp_sep_views(ag.bpm, xc, yc, A);
     if (!bsv.empty()) {
         ALLOC(ag.d_bsv, sizeof(BpSepView) * A);
         if (hipMemcpyAsync(ag.d_bsv, bsv.data(), sizeof(BpSepView) * A, hipMemcpyHostToDevice, ctx->stream) !=
@@ -614,8 +731,49 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ALLOC(ag.d_x, es * ctx->N);
     ALLOC(ag.d_w, es * ctx->N);
     if (ag.fast != 0) ALLOC(ag.d_tmp, es * ctx->N);
+    if (ag.col.on) {
+        // strips: columns grouped by k0 = floor(v) in [-1, nv - 1], each
+        // strip in memory order; v is the reference's chain with yc3 = 0
+        const mtr_volume_size &vb = ctx->vol;
+        const int nstrip = prj->Y + 1;
+        std::vector<int> cnt(nstrip + 1, 0);
+        std::vector<int> key((size_t)vb.X * vb.Y);
+        std::vector<double> dvs((size_t)vb.X * vb.Y);
+        for (int Y = vb.Ystart; Y < vb.Yend; Y++)
+            for (int X = vb.Xstart; X < vb.Xend; X++) {
+                const double v = std::fma(yc[2], (double)Y, std::fma(yc[1], (double)X, yc[0])) - 1.0;
+                const double k0 = std::floor(v);
+                const size_t c = (size_t)(Y - vb.Ystart) * vb.X + (X - vb.Xstart);
+                key[c] = (k0 < -1.0 || k0 > prj->Y - 1) ? -1 : (int)k0 + 1;
+                dvs[c] = v - k0;
+                if (key[c] >= 0) cnt[key[c] + 1]++;
+            }
+        for (int q = 0; q < nstrip; q++) cnt[q + 1] += cnt[q];
+        std::vector<int2> xy(std::max(cnt[nstrip], 1));
+        std::vector<float2> wts(std::max(cnt[nstrip], 1));
+        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+        for (int Y = vb.Ystart; Y < vb.Yend; Y++)
+            for (int X = vb.Xstart; X < vb.Xend; X++) {
+                const size_t c = (size_t)(Y - vb.Ystart) * vb.X + (X - vb.Xstart);
+                if (key[c] < 0) continue;
+                const int at = pos[key[c]]++;
+                xy[at] = make_int2(X, Y);
+                wts[at] = make_float2((float)(1.0 - dvs[c]), (float)dvs[c]);
+            }
+        ALLOC(ag.d_col_xy, sizeof(int2) * xy.size());
+        ALLOC(ag.d_col_w, sizeof(float2) * wts.size());
+        ALLOC(ag.d_strip, sizeof(int) * cnt.size());
+        ALLOC(ag.d_pm, sizeof(float) * P);
+        ALLOC(ag.d_pn, sizeof(float) * P);
+        if (hipMemcpy(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ag.d_col_w, wts.data(), sizeof(float2) * wts.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(ag.d_strip, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return undo(fail(MTR_ERR_HIP, "column table upload failed"));
+    }
     ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
-    ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag))), 1));
+    ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag))),
+                                                           fp_col_res_blocks(make_dev(ctx, ag))),
+                                                  1));
     ALLOC(ag.d_scalar, sizeof(double));
     if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
@@ -863,7 +1021,8 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ag : ctx->agents) {
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
-                        ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std})
+                        ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
+                        (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
@@ -929,9 +1088,23 @@ int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size
     if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
     const size_t es = elem_size(ctx->precision);
     const size_t P = (size_t)prj->X * prj->Y * prj->AngN;
+    const size_t N = (size_t)ctx->N;
+    const size_t cols = (size_t)ctx->vol.X * ctx->vol.Y;
     const size_t nfp = (size_t)((prj->X + 63) / 64 + 16) * ((prj->Y + 3) / 4 + 16) * prj->AngN;
-    *bytes = sizeof(ViewCoef) * prj->AngN + sizeof(FpPlan) * prj->AngN + es * P /*p*/ + 4 * P /*c*/ +
-             es * P /*r*/ + 2 * es * (size_t)ctx->N /*x, w*/ + 8 * nfp;
+    // geometry tables, p (plus a converted copy), c, r, the partial rows of
+    // the column plan, x, w and a canonical copy of x, column tables, FP64
+    // partial sums
+    *bytes = (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj->AngN + 2 * 4 * P + 4 * P + es * P +
+             2 * 4 * P + 3 * es * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp;
+    return MTR_OK;
+}
+
+int mtr_agent_kernels(const mtr_ctx *ctx, int agent, int *fp_kind, int *bp_kind)
+{
+    MTR_TRY(check_agent(const_cast<mtr_ctx *>(ctx), agent));
+    const Agent &ag = ctx->agents[agent];
+    if (fp_kind) *fp_kind = ag.col.on ? MTR_FPK_COLUMN : (ag.fp_nc ? MTR_FPK_STREAM : MTR_FPK_GENERAL);
+    if (bp_kind) *bp_kind = ag.col.on ? MTR_BPK_COLUMN : (ag.bpm.sep ? MTR_BPK_SEPARABLE : MTR_BPK_GENERAL);
     return MTR_OK;
 }
 

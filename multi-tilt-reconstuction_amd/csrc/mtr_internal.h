@@ -70,6 +70,16 @@ struct AgentDev {
     void *r;               // residual, strides pk/pi
     void *x;               // working volume, strides sx/sy/sz
     double *partial;       // per-block FP64 partial sums
+    // Column plan (v independent of Z and identical in every view; the
+    // working volume is Z-contiguous): the (X, Y) columns grouped into
+    // strips by k0 = floor(v), strip s holding k0 = s - 1.
+    const int2 *col_xy;    // absolute (X, Y) of each column, strip by strip
+    const float2 *col_w;   // row weights (1 - dv, dv) of each column, formed in FP64
+    const int *strip;      // [nstrip + 1] first column of each strip
+    int nstrip;            // nv + 1 (k0 = -1 .. nv - 1)
+    int col_p1y;           // back-projection tiles: lanes along Y (1) or X (0)
+    int col_pitch;         // back-projection LDS pitch (pairs per staged row)
+    float *pm, *pn;        // FP partial rows [A][nv][nu]: from strip k and strip k - 1
 };
 
 }  // namespace mtr

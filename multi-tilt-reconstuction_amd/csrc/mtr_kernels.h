@@ -20,6 +20,24 @@ int bp_sep_waves();
 hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
                          int P1, int P2, int rw, hipStream_t st);
 
+// Column plan (v independent of Z and of the view; Z-contiguous volume).
+// Forward projection into partial rows pm/pn (unit: x = 1), then the sum
+// and residual (mode 0) or the normaliser into c_out (mode 2).
+int fp_col_blocks(const AgentDev &ag);
+int fp_col_max_nu();
+hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+                         hipStream_t st);
+int fp_col_res_blocks(const AgentDev &ag);
+hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
+                             hipStream_t st);
+// Back-projection with runs of rw (16 or 32) voxels along Z; the staged
+// footprint (rows x col_pitch pairs) must fit col_elems().
+int bp_col_blocks(const AgentDev &ag, int rw);
+int bp_col_waves();
+int col_elems();
+hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+                         hipStream_t st);
+
 // Forward projection of nag agents (device table d_ags), mode 0 = fused
 // residual, 1 = raw s/c into s_out/c_out, 2 = normaliser into c_out.
 hipError_t launch_fp(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, bool lanes_k,

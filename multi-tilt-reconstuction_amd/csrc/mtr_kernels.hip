@@ -93,6 +93,11 @@ __device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
     return fma(c[3], (double)Z, fma(c[2], (double)Y, fma(c[1], (double)X, c[0]))) - 1.0;
 }
 
+// Tent weight max(0, 1 - |t|) of a signed distance t (in detector
+// columns), formed in FP64 and rounded once, so small weights keep their
+// relative precision.
+__device__ __forceinline__ float tent(double t) { return (float)fmax(1.0 - fabs(t), 0.0); }
+
 // Select among three per-axis values without a dynamically indexed array.
 template <class V>
 __device__ __forceinline__ V sel3(V x, V y, V z, int axis)
@@ -208,14 +213,13 @@ __device__ __forceinline__ void sep_line_load(const T *x, int64_t lb, int64_t sM
 {
     const double mf = floor(mc - pl.Em) + 1.0;
     int m = (int)mf;
-    float t = (float)(aM * (mf - mc));
+    const double t0 = aM * (mf - mc);
 #pragma unroll
     for (int j = 0; j < NC; j++) {
         const bool ok = (unsigned)(m - Mlo) <= mspan;
-        wv[j] = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+        wv[j] = ok ? tent(fma((double)j, aM, t0)) : 0.0f;
         if (!unit_x) xv[j] = ldf<T>(x, lb + (ok ? m : Mlo) * sM);
         m++;
-        t += aMf;
     }
 }
 
@@ -259,11 +263,11 @@ __device__ void sep_lines_any(FpAcc &acc, const T *x, int64_t base, int64_t sL, 
         const double mc = ms + l * pl.ml;
         const double mf = floor(mc - pl.Em) + 1.0;
         int m = (int)mf;
-        float t = (float)(aM * (mf - mc));
+        const double t0 = aM * (mf - mc);
         const int64_t lb = base + l * sL;
-        for (int j = 0; j < pl.ncand; j++, m++, t += aMf) {
+        for (int j = 0; j < pl.ncand; j++, m++) {
             const bool ok = (unsigned)(m - Mlo) <= mspan;
-            const float wgt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+            const float wgt = ok ? tent(fma((double)j, aM, t0)) : 0.0f;
             if (NEED_C) acc.c += wgt;
             if (UNIT_X) acc.s += wgt;
             else acc.s = fmaf(wgt, ldf<T>(x, lb + (ok ? m : Mlo) * sM), acc.s);
@@ -386,9 +390,13 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     // one XCD are all views of a few adjacent columns of one plane group and
     // the voxels they share stay in that XCD's L2 across views.
     const int nbt = nti * ntk * ag.A, per = (nbt + 7) / 8;
-    if (blockIdx.x >= (unsigned)(per * 8)) return;
+    if (blockIdx.x >= (unsigned)(per * 8)) return;  // beyond this agent's partial sums
     int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (b >= nbt) return;
+    if (b >= nbt) {
+        // padding block of the 8-way split: its partial sum is still read
+        if (threadIdx.x == 0) ag.partial[blockIdx.x] = 0.0;
+        return;
+    }
     const int a = b % ag.A;
     b /= ag.A;
     const int bi = b % nti;
@@ -434,14 +442,9 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                 const double mc = ms + l * pl.ml;
                 const double mf = floor(mc - pl.Em) + 1.0;
                 const int m = (int)mf + j;
-                // same FP32 steps as the per-lane path: t0, t0 + aM, t0 + aM + aM, ...
-                const float aMf = (float)aM;
-                float t = (float)(aM * (mf - mc));
-#pragma unroll
-                for (int q = 1; q < NC; q++)
-                    if (q <= j) t += aMf;
+                // same weight as the per-lane path
                 const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
-                const float wt = ok ? fmaxf(1.0f - fabsf(t), 0.0f) : 0.0f;
+                const float wt = ok ? tent(fma((double)j, aM, aM * (mf - mc))) : 0.0f;
                 // offset from the first plane of the box, >= 0
                 const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) : 0;
                 const int64_t lm0 = (int64_t)Llo * sL + (int64_t)Mlo * sM;  // folded into the base below
@@ -829,6 +832,353 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+// ------------------------------------------------------------------------
+// Column plan: every view has v independent of Z and the same v (slab
+// geometry with any in-plane axis), so each (X, Y) column lands on the two
+// detector rows k0 = floor(v), k0 + 1 with weights (1 - dv, dv) in every
+// view, while u runs linearly along the column (step xc3 per plane).  The
+// working volume is Z-contiguous.
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One step of a segmented inclusive scan of (a, b) over the lanes of a
+// wave: a lane adds the value of the lane CTRL names (DPP) when both carry
+// the same key.  Keys are monotone along the lanes, so equal keys at two
+// lanes mean every lane between them shares the key.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_step(float &a, float &b, int key)
+{
+    const float a2 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), CTRL, ROWS, 0xf, false));
+    const float b2 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), CTRL, ROWS, 0xf, false));
+    const int k2 = __builtin_amdgcn_update_dpp((int)0x80000000, key, CTRL, ROWS, 0xf, false);
+    if (k2 == key) {
+        a += a2;
+        b += b2;
+    }
+}
+
+// Segmented inclusive scan: Hillis-Steele inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8; `steps` of them suffice for segments of at most
+// 2^steps lanes), then the row carries (row_bcast 15 into rows 1 and 3,
+// row_bcast 31 into rows 2 and 3).  The last lane of a segment ends with
+// the segment's sums.
+__device__ __forceinline__ void seg_scan(float &a, float &b, int key, int steps)
+{
+    if (steps >= 1) seg_step<0x111, 0xf>(a, b, key);
+    if (steps >= 2) seg_step<0x112, 0xf>(a, b, key);
+    if (steps >= 3) seg_step<0x114, 0xf>(a, b, key);
+    if (steps >= 4) seg_step<0x118, 0xf>(a, b, key);
+    seg_step<0x142, 0xa>(a, b, key);
+    seg_step<0x143, 0xc>(a, b, key);
+}
+
+// Forward projection: one wave per (strip, view); a strip is the set of
+// columns sharing k0.  The wave walks the strip's columns in a fixed
+// order; for each column its 64 lanes take 64 consecutive planes (one
+// coalesced load), form the taps (1 - du) x into detector column floor(u)
+// and du x into floor(u) + 1, and the segmented scan sums the lanes that
+// share a detector column.  The last lane of each segment adds the two
+// sums, weighted (1 - dv) and dv, into the wave's two LDS detector rows:
+// within one instruction every address is touched by one lane and only
+// this wave touches these rows, so the summation order is fixed.  The rows
+// are written out as partial rows pm (row k0) and pn (row k0 + 1);
+// k_fp_col_res adds them.  UNIT: x taken as 1 (the normaliser).
+constexpr int kColWaves = 4;
+
+template <class T, bool UNIT>
+__global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__restrict__ ags)
+{
+    extern __shared__ float col_rows[];
+    const AgentDev &ag = ags[blockIdx.y];
+    const int nvg = (ag.A + kColWaves - 1) / kColWaves;
+    // XCD-aware order, strip-major: the view groups of one strip run on
+    // one XCD back to back and share the strip's columns in its L2
+    const int nb = ag.nstrip * nvg, per = (nb + 7) / 8;
+    if (blockIdx.x >= (unsigned)(per * 8)) return;
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (b >= nb) return;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int s = b / nvg, a = (b - s * nvg) * kColWaves + wid;
+    if (a >= ag.A) return;  // no block-wide barrier below
+    const int nu = ag.nu, nv = ag.nv, L = nu + 2;
+    float *rm = col_rows + wid * 2 * L, *rn = rm + L;
+    for (int q = lane; q < 2 * L; q += 64) rm[q] = 0.0f;
+    wave_lds_sync();
+    const ViewCoef vc = ag.coef[a];
+    const double c0 = vc.c[0], c1 = vc.c[1], c2 = vc.c[2], c3 = vc.c[3];
+    // longest run of consecutive planes sharing floor(u): 1/|xc3| + 1
+    const double ac3 = fabs(c3);
+    const int maxrun = ac3 > 0.0 ? (int)fmin(64.0, floor(1.0 / ac3) + 1.0) : 64;
+    const int steps = maxrun <= 1 ? 0 : 32 - __clz(maxrun - 1);
+    const T *x = reinterpret_cast<const T *>(ag.x);
+    const int nz = ag.Z, zs = ag.Zs;
+    const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
+    for (int j = j0; j < j1; j++) {
+        const int2 xy = ag.col_xy[j];
+        const float2 cw = ag.col_w[j];
+        const float wm = cw.x, wn = cw.y;
+        const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
+        const int64_t coff = (int64_t)(xy.x - ag.Xs) * ag.sx + (int64_t)(xy.y - ag.Ys) * ag.sy;
+        for (int zc = 0; zc < nz; zc += 64) {
+            const int zi = zc + lane;
+            const bool ok = zi < nz;
+            float xv = 0.0f;
+            if (ok) xv = UNIT ? 1.0f : ldf<T>(x, coff + zi);
+            // the reference's chain: fma(xc3, Z, fma(xc2, Y, fma(xc1, X, xc0))) - 1
+            const double u = fma(c3, (double)(zs + zi), bxy) - 1.0;
+            const double fl = floor(u);
+            const int key = ok ? (int)fl : 0x40000000 + lane;
+            // tap weights rounded from FP64 (no cancellation in 1 - du)
+            const double dd = u - fl;
+            float ea = (float)(1.0 - dd) * xv, eb = (float)dd * xv;
+            seg_scan(ea, eb, key, steps);
+            const int nkey = __shfl_down(key, 1, 64);
+            const bool lead = ok && (lane == 63 || nkey != key);
+            const int bi = key + 1;  // LDS slot of detector column key (column -1 -> slot 0)
+            if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
+                rm[bi] += wm * ea;
+                rn[bi] += wn * ea;
+            }
+            wave_lds_sync();
+            if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
+                rm[bi + 1] += wm * eb;
+                rn[bi + 1] += wn * eb;
+            }
+            wave_lds_sync();
+        }
+    }
+    const int k0 = s - 1;
+    const int64_t row = ((int64_t)a * nv + k0) * nu;
+    for (int i = lane; i < nu; i += 64) {
+        if (k0 >= 0) ag.pm[row + i] = rm[i + 1];
+        if (k0 + 1 < nv) ag.pn[row + nu + i] = rn[i + 1];
+    }
+}
+
+// s = pm + pn; MODE 0: residual r = p - (c != 0 ? s/c : s) with FP64
+// per-block sums of r^2; MODE 2: the normaliser c = s into c_out.
+template <class T, int MODE>
+__global__ __launch_bounds__(256) void k_fp_col_res(const AgentDev *__restrict__ ags, float *__restrict__ c_out)
+{
+    const AgentDev &ag = ags[blockIdx.y];
+    const int64_t P = (int64_t)ag.A * ag.nu * ag.nv;
+    double r2 = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < P; n += (int64_t)gridDim.x * 256) {
+        const float sv0 = ag.pm[n] + ag.pn[n];
+        if (MODE == 2) {
+            c_out[n] = sv0;
+        } else {
+            const float cc = ag.c[n];
+            float sv = sv0;
+            if (cc != 0.0f) sv = sv / cc;
+            const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), n) - sv;
+            stf<T>(reinterpret_cast<T *>(ag.r), n, rv);
+            const double rs = (double)rnd<T>(rv);
+            r2 += rs * rs;
+        }
+    }
+    if (MODE == 0) {
+        __shared__ double sh[4];
+        const double t = block_sum<256>(r2, sh);
+        if (threadIdx.x == 0) ag.partial[blockIdx.x] = t;
+    }
+}
+
+// Back-projection of column-plan agents.  Block: 64 lanes along P1 (X, or
+// Y when col_p1y) x 8 waves along the other axis; each thread owns a run
+// of RW voxels along Z.  All voxels of a column read rows k0, k0 + 1 with
+// weights (1 - dv, dv) in every view, so the tile's rows are fixed for the
+// launch; per view the block stages them over the tile's u range as
+// (r[i], r[i+1] - r[i]) pairs (zero outside the detector, double
+// buffered), and voxel t of the run gathers both rows at u0 + t xc3.
+// The normaliser C of a voxel is (wk0 + wk1) times the summed u weights:
+// one per view less the weight of taps that fall off the detector's sides.
+constexpr int kColBpWaves = 8;
+constexpr int kColElems = 7 * 64 * kColBpWaves;  // staged pairs per view (rows x pitch)
+
+int col_elems() { return kColElems; }
+
+template <class T, int RW, bool BPT>
+__global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__restrict__ ags, float lambda)
+{
+    constexpr int NT = 64 * kColBpWaves, kPer = kColElems / NT;
+    constexpr int kStage = 2 * kColElems, kEpi = kColBpWaves * (64 * 17 + 64) / 2;
+    __shared__ float2 lds_raw[kStage > kEpi ? kStage : kEpi];
+    const AgentDev &ag = ags[blockIdx.y];
+    const bool p1y = ag.col_p1y != 0;
+    const int pitch = ag.col_pitch;
+    const int n1 = p1y ? ag.Y : ag.X, n2 = p1y ? ag.X : ag.Y;
+    const int s1 = p1y ? ag.Ys : ag.Xs, s2 = p1y ? ag.Xs : ag.Ys;
+    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kColBpWaves - 1) / kColBpWaves, ntz = (ag.Z + RW - 1) / RW;
+    const int nb = nt1 * nt2 * ntz, per = (nb + 7) / 8;
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
+    const int bz = b % ntz, b1 = (b / ntz) % nt1, b2 = b / (ntz * nt1);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kColBpWaves, Z0 = ag.Zs + bz * RW;
+    const int p1 = P10 + lane, p2 = P20 + wid;
+    const int e1 = min(s1 + n1 - 1, P10 + 63), e2 = min(s2 + n2 - 1, P20 + kColBpWaves - 1);
+    const int ew = min(RW, ag.Ze - Z0) - 1;
+    const int nu = ag.nu, nv = ag.nv;
+    const double y0 = ag.coef[0].c[4], yx = ag.coef[0].c[5], yy = ag.coef[0].c[6];
+    // the tile's rows: v is affine in (X, Y), extremes at the corners
+    double vmin = 1e300, vmax = -1e300;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c1 = (q & 1) ? e1 : P10, c2 = (q & 2) ? e2 : P20;
+        const double vq = fma(yy, (double)(p1y ? c1 : c2), fma(yx, (double)(p1y ? c2 : c1), y0)) - 1.0;
+        vmin = fmin(vmin, vq);
+        vmax = fmax(vmax, vq);
+    }
+    const int klo = (int)floor(vmin), nrows = (int)floor(vmax) - klo + 2;
+    // this thread's column (clamped into the box; stores are masked)
+    const int q1 = min(p1, e1), q2 = min(p2, e2);
+    const int X = p1y ? q2 : q1, Y = p1y ? q1 : q2;
+    const double v = fma(yy, (double)Y, fma(yx, (double)X, y0)) - 1.0;
+    const double vf = floor(v);
+    const int k0 = (int)vf;
+    const float wk0 = (unsigned)k0 < (unsigned)nv ? (float)(1.0 - (v - vf)) : 0.0f;
+    const float wk1 = (unsigned)(k0 + 1) < (unsigned)nv ? (float)(v - vf) : 0.0f;
+    const int lr = (k0 - klo) * pitch;
+    const auto src = gview(reinterpret_cast<const T *>(BPT ? ag.p : ag.r));
+    const int64_t plane = (int64_t)nu * nv;
+
+    float S[RW], D[RW];  // D: summed u weight of taps off the detector
+#pragma unroll
+    for (int t = 0; t < RW; t++) S[t] = D[t] = 0.0f;
+    float2 pre[kPer];
+
+    // detector columns of the tile in view a: u at the tile origin plus
+    // the host's extreme offsets over the tile, one column of margin
+    const int Xt = p1y ? P20 : P10, Yt = p1y ? P10 : P20;
+    auto window = [&](int a, int &fi0, int &fi1) {
+        const BpSepView bv = ag.bsv[a];
+        const ViewCoef vc = ag.coef[a];
+        const double ut = fma(vc.c[3], (double)Z0, fma(vc.c[2], (double)Yt, fma(vc.c[1], (double)Xt, vc.c[0]))) - 1.0;
+        fi0 = (int)floor(ut + bv.umin) - 1;
+        fi1 = (int)floor(ut + bv.umax) + 1;
+    };
+    auto fetch = [&](int a, int fi0, int fi1) {
+        const int width = fi1 - fi0 + 1;
+        const uint32_t vbase = (uint32_t)(plane * a);
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int e = threadIdx.x + NT * q;
+            const int rr = e / pitch, cc = e - rr * pitch;
+            const int gk = klo + rr, gi = fi0 + cc;
+            const bool ok = rr < nrows && cc < width && (unsigned)gk < (unsigned)nv;
+            const bool ok0 = ok && (unsigned)gi < (unsigned)nu, ok1 = ok && (unsigned)(gi + 1) < (unsigned)nu;
+            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * (uint32_t)nu;
+            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1));
+            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1));
+            const float r0 = ok0 ? v0 : 0.0f, r1 = ok1 ? v1 : 0.0f;
+            pre[q] = make_float2(r0, r1 - r0);
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kPer; q++) lds_raw[buf * kColElems + threadIdx.x + NT * q] = pre[q];
+    };
+
+    int fi0, fi1;
+    window(0, fi0, fi1);
+    fetch(0, fi0, fi1);
+    commit(0);
+    __syncthreads();
+    for (int a = 0; a < ag.A; a++) {
+        const int cfi0 = fi0, cfi1 = fi1;
+        if (a + 1 < ag.A) {
+            window(a + 1, fi0, fi1);
+            fetch(a + 1, fi0, fi1);
+        }
+        // u of every voxel by the reference's FP64 chain (so a voxel's
+        // value does not depend on where its run starts)
+        const ViewCoef vc = ag.coef[a];
+        const double bxy = fma(vc.c[2], (double)Y, fma(vc.c[1], (double)X, vc.c[0]));
+        const double c3 = vc.c[3];
+        const float2 *row0 = lds_raw + (a & 1) * kColElems + lr;
+        const float2 *row1 = row0 + pitch;
+        const int lmax = pitch - 1;
+        if (cfi0 >= 0 && cfi1 + 1 <= nu - 1) {
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                const double u = fma(c3, (double)(Z0 + t), bxy) - 1.0;
+                const double fl = floor(u);
+                const float du = (float)(u - fl);
+                const int li = min(max((int)fl - cfi0, 0), lmax);
+                const float2 qa = row0[li], qb = row1[li];
+                S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
+                S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < RW; t++) {
+                const double u = fma(c3, (double)(Z0 + t), bxy) - 1.0;
+                const double fl = floor(u);
+                const float du = (float)(u - fl);
+                const int i0 = (int)fl;
+                const int li = min(max(i0 - cfi0, 0), lmax);
+                const float2 qa = row0[li], qb = row1[li];
+                S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
+                S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
+                // weight of the taps that fall off the detector's sides
+                const bool v0 = (unsigned)i0 < (unsigned)nu, v1 = (unsigned)(i0 + 1) < (unsigned)nu;
+                D[t] += v0 ? (v1 ? 0.0f : du) : (v1 ? 1.0f - du : 1.0f);
+            }
+        }
+        if (a + 1 < ag.A) commit((a + 1) & 1);
+        __syncthreads();
+    }
+    // Epilogue: q = S/C per voxel and a bit per voxel for C != 0, then the
+    // runs are stored transposed through LDS (16 lanes per 16 consecutive
+    // voxels of one run) as in the separable back-projection.
+    float *ep = reinterpret_cast<float *>(lds_raw) + wid * (64 * 17 + 64);
+    int *mk = reinterpret_cast<int *>(ep + 64 * 17);
+    const float kw = wk0 + wk1;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        const float C = kw * ((float)ag.A - D[t]);
+        S[t] = C != 0.0f ? S[t] / C : 0.0f;
+        mask |= (C != 0.0f ? 1u : 0u) << t;
+    }
+    mk[lane] = (int)mask;
+    T *x = reinterpret_cast<T *>(ag.x);
+    const int64_t sP1 = p1y ? ag.sy : ag.sx;
+    const int X0 = p1y ? p2 : P10, Y0 = p1y ? P10 : p2;
+    const int64_t jb = (int64_t)(X0 - ag.Xs) * ag.sx + (int64_t)(Y0 - ag.Ys) * ag.sy + (int64_t)(Z0 - ag.Zs) * ag.sz;
+    const bool wave_ok = p2 <= e2;
+    constexpr int CH = RW < 16 ? RW : 16, LPI = 64 / CH;
+    static_assert(RW % CH == 0 && 64 % CH == 0, "run length must divide into chunks");
+#pragma unroll
+    for (int h = 0; h < RW / CH; h++) {
+#pragma unroll
+        for (int t = 0; t < CH; t++) ep[lane * 17 + t] = S[CH * h + t];
+        wave_lds_sync();
+        const int tt = lane % CH, tw = CH * h + tt;
+#pragma unroll 4
+        for (int i = 0; i < CH; i++) {
+            const int line = i * LPI + lane / CH;
+            const float qv = ep[line * 17 + tt];
+            const bool cnz = (mk[line] >> tw) & 1;
+            if (wave_ok && tw <= ew && P10 + line <= e1) {
+                const int64_t j = jb + line * sP1 + tw * ag.sz;
+                if (BPT) {
+                    stf<T>(x, j, qv);
+                } else if (cnz) {
+                    stf<T>(x, j, fmaf(qv, lambda, ldf<T>(x, j)));
+                }
+            }
+        }
+        wave_lds_sync();
     }
 }
 
@@ -1483,6 +1833,91 @@ hipError_t launch_permute(int prec, const void *src, void *dst, int nX, int nY, 
         hipLaunchKernelGGL(k_permute<float>, dim3((unsigned)nb), dim3(256), 0, st, (const float *)src, (float *)dst,
                            pa);
     return hipGetLastError();
+}
+
+int fp_col_blocks(const AgentDev &ag)
+{
+    const int nb = ag.nstrip * ((ag.A + kColWaves - 1) / kColWaves);
+    return ((nb + 7) / 8) * 8;
+}
+
+int fp_col_max_nu() { return (int)(65536 / (kColWaves * 2 * sizeof(float))) - 2; }
+
+template <class T>
+static hipError_t launch_fp_col_t(const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+                                  hipStream_t st)
+{
+    const size_t shm = (size_t)kColWaves * 2 * (nu_max + 2) * sizeof(float);
+    dim3 grid(max_blocks, nag), block(64 * kColWaves);
+    if (unit) hipLaunchKernelGGL((k_fp_col<T, true>), grid, block, shm, st, d_ags);
+    else hipLaunchKernelGGL((k_fp_col<T, false>), grid, block, shm, st, d_ags);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+                         hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_fp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, nu_max, unit, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_col_t<__half>(d_ags, nag, max_blocks, nu_max, unit, st);
+    return launch_fp_col_t<float>(d_ags, nag, max_blocks, nu_max, unit, st);
+}
+
+int fp_col_res_blocks(const AgentDev &ag)
+{
+    const int64_t P = (int64_t)ag.A * ag.nu * ag.nv;
+    return (int)std::min<int64_t>((P + 255) / 256, 2048);
+}
+
+template <class T>
+static hipError_t launch_fp_col_res_t(const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
+                                      hipStream_t st)
+{
+    dim3 grid(blocks, nag), block(256);
+    if (mode == 2) hipLaunchKernelGGL((k_fp_col_res<T, 2>), grid, block, 0, st, d_ags, c_out);
+    else hipLaunchKernelGGL((k_fp_col_res<T, 0>), grid, block, 0, st, d_ags, c_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
+                             hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_fp_col_res_t<__hip_bfloat16>(d_ags, nag, blocks, mode, c_out, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_col_res_t<__half>(d_ags, nag, blocks, mode, c_out, st);
+    return launch_fp_col_res_t<float>(d_ags, nag, blocks, mode, c_out, st);
+}
+
+int bp_col_blocks(const AgentDev &ag, int rw)
+{
+    const int n1 = ag.col_p1y ? ag.Y : ag.X, n2 = ag.col_p1y ? ag.X : ag.Y;
+    const int nb = ((n1 + 63) / 64) * ((n2 + kColBpWaves - 1) / kColBpWaves) * ((ag.Z + rw - 1) / rw);
+    return ((nb + 7) / 8) * 8;
+}
+
+int bp_col_waves() { return kColBpWaves; }
+
+template <class T, int RW>
+static void launch_bp_col_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, hipStream_t st)
+{
+    if (bpt) hipLaunchKernelGGL((k_bp_col<T, RW, true>), grid, dim3(64 * kColBpWaves), 0, st, d_ags, lambda);
+    else hipLaunchKernelGGL((k_bp_col<T, RW, false>), grid, dim3(64 * kColBpWaves), 0, st, d_ags, lambda);
+}
+
+template <class T>
+static hipError_t launch_bp_col_t(const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+                                  hipStream_t st)
+{
+    dim3 grid(max_blocks, nag);
+    if (rw == 32) launch_bp_col_rw<T, 32>(grid, d_ags, bpt, lambda, st);
+    else launch_bp_col_rw<T, 16>(grid, d_ags, bpt, lambda, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+                         hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_bp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
+    if (prec == MTR_PREC_FP16) return launch_bp_col_t<__half>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
+    return launch_bp_col_t<float>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
 }
 
 hipError_t launch_reduce(const double *part, int64_t n, double *out, hipStream_t st)

@@ -136,6 +136,49 @@ def test_line_golden_on_gpu(mtr, ctx):
     assert xt.ravel().tolist() == g["bpt"]
 
 
+# Resident-agent kernels (selected from the geometry) against the oracle:
+# streaming FP / separable BP for row-separable agents, the column plan for
+# oblique in-plane axes, the general kernels otherwise.
+AGENT_GEOMS = GEOMS + [
+    ("col60_deep", dict(phis=[60.0], X=40, Y=36, Z=75)),           # two 64-plane chunks per column
+    ("col135_small_det", dict(phis=[135.0], X=30, Y=30, Z=20, nu=17, nv=13)),  # footprint off the detector
+    ("col45_origin", dict(phis=[45.0], origin=(3, -2, 5))),
+    ("col75_wide", dict(phis=[75.0], X=150, Y=21, Z=9)),            # several back-projection tiles
+]
+
+
+def expected_kernels(mtr, name):
+    if name in ("axisZ", "slab0"):
+        return mtr.FPK_STREAM, mtr.BPK_SEPARABLE
+    if name in ("slab90", "origin"):
+        # planes of the box fall off the detector's rows: the separable
+        # back-projection needs every plane on a row
+        return mtr.FPK_STREAM, mtr.BPK_GENERAL
+    # oblique axes, and the 15 x 9 detector whose separable rays would need
+    # more than four candidates per line
+    return mtr.FPK_COLUMN, mtr.BPK_COLUMN
+
+
+@pytest.mark.parametrize("name,kw", AGENT_GEOMS, ids=[g[0] for g in AGENT_GEOMS])
+def test_agent_kernels_match_oracle(mtr, orc, name, kw):
+    pb = problem(mtr, **kw)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    say("agent kernels", name)
+    with load_ctx(mtr, pb) as c:
+        say("kernels", c.agent_kernels(0))
+        assert c.agent_kernels(0) == expected_kernels(mtr, name)
+        xb = c.bpt(0)
+        x_ref = orc.bpt(prj, vol, xc, yc, p)
+        assert orc.rel_l2(xb, x_ref) < OP_TOL
+        x1 = c.sirt(0, 1, 0.7)
+        assert orc.rel_l2(x1, orc.sirt(prj, vol, xc, yc, p, x_ref, 1, 0.7)) < OP_TOL
+        # FP64 norm of the residual formed by the forward projection of the
+        # volume that iteration started from
+        s0, c0 = orc.fp(prj, vol, xc, yc, xb)
+        r = orc.residual(p, s0, c0).astype(np.float64)
+        assert c.residual_norm2(0) == pytest.approx(float((r * r).sum()), rel=1e-5)
+
+
 def load_ctx(mtr, pb, precision=0):
     c = mtr.Context(0, precision)
     c.set_volume(pb.vol)
