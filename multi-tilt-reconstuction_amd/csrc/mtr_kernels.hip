@@ -93,6 +93,9 @@ __device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
     return fma(c[3], (double)Z, fma(c[2], (double)Y, fma(c[1], (double)X, c[0]))) - 1.0;
 }
 
+// Index of the calling wave in its block, as a wave-uniform (scalar) value.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // Tent weight max(0, 1 - |t|) of a signed distance t (in detector
 // columns), formed in FP64 and rounded once, so small weights keep their
 // relative precision.
@@ -401,7 +404,7 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     b /= ag.A;
     const int bi = b % nti;
     const int bk = b / nti;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = wave_id();
     const int i = bi * 4 + wid;
     const int k = bk * 64 + lane;
     const FpPlan pl = ag.plan[a];
@@ -696,7 +699,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
     const int b1 = b % nt1, b2 = (b / nt1) % nt2, bw = b / (nt1 * nt2);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = wave_id();
     const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kSepWaves, W0 = sw + bw * RW;
     const int p1 = P10 + lane, p2 = P20 + wid;
     const int ew = min(RW, sw + nw - W0) - 1;
@@ -849,35 +852,88 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One step of a segmented inclusive scan of (a, b) over the lanes of a
-// wave: a lane adds the value of the lane CTRL names (DPP) when both carry
-// the same key.  Keys are monotone along the lanes, so equal keys at two
-// lanes mean every lane between them shares the key.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ void seg_step(float &a, float &b, int key)
+// DPP move of a 32-bit value.  dpp_z: lanes the pattern does not reach
+// read 0 (for full row masks); dpp_old: they keep `old`.
+template <int CTRL>
+__device__ __forceinline__ int dpp_z(int v)
 {
-    const float a2 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a), CTRL, ROWS, 0xf, false));
-    const float b2 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(b), CTRL, ROWS, 0xf, false));
-    const int k2 = __builtin_amdgcn_update_dpp((int)0x80000000, key, CTRL, ROWS, 0xf, false);
-    if (k2 == key) {
-        a += a2;
-        b += b2;
-    }
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_old(int old, int v)
+{
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
 }
 
-// Segmented inclusive scan: Hillis-Steele inside each row of 16 lanes
-// (row_shr 1, 2, 4, 8; `steps` of them suffice for segments of at most
-// 2^steps lanes), then the row carries (row_bcast 15 into rows 1 and 3,
-// row_bcast 31 into rows 2 and 3).  The last lane of a segment ends with
-// the segment's sums.
-__device__ __forceinline__ void seg_scan(float &a, float &b, int key, int steps)
+// Keys of the segmented scan are biased so that no key is 0, the value a
+// lane reads when a DPP pattern points outside the wave or row.
+constexpr int kKeyBias = 1 << 24;
+
+// One Hillis-Steele step of a segmented inclusive scan of (a, b): a lane
+// adds the value of the lane the DPP pattern CTRL names when both carry
+// the same key.  Keys are monotone along the lanes, so equal keys at two
+// lanes mean every lane between them shares the key.
+template <int CTRL>
+__device__ __forceinline__ void seg_step(float &a, float &b, int key)
 {
-    if (steps >= 1) seg_step<0x111, 0xf>(a, b, key);
-    if (steps >= 2) seg_step<0x112, 0xf>(a, b, key);
-    if (steps >= 3) seg_step<0x114, 0xf>(a, b, key);
-    if (steps >= 4) seg_step<0x118, 0xf>(a, b, key);
-    seg_step<0x142, 0xa>(a, b, key);
-    seg_step<0x143, 0xc>(a, b, key);
+    const bool same = dpp_z<CTRL>(key) == key;
+    const float a2 = __int_as_float(dpp_z<CTRL>(__float_as_int(a)));
+    const float b2 = __int_as_float(dpp_z<CTRL>(__float_as_int(b)));
+    a += same ? a2 : 0.0f;
+    b += same ? b2 : 0.0f;
+}
+// Row-carry step (row_bcast): rows outside ROWS keep their values.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void seg_carry(float &a, float &b, int key)
+{
+    const bool same = dpp_old<CTRL, ROWS>(0, key) == key;
+    const float a2 = __int_as_float(dpp_old<CTRL, ROWS>(0, __float_as_int(a)));
+    const float b2 = __int_as_float(dpp_old<CTRL, ROWS>(0, __float_as_int(b)));
+    a += same ? a2 : 0.0f;
+    b += same ? b2 : 0.0f;
+}
+
+// Adds the original values of the previous R lanes (wave_shr 1 applied
+// r times, across rows) that share the lane's key, in lane order.
+template <int R>
+__device__ __forceinline__ void seg_linear(float &a, float &b, int key)
+{
+    float sa = a, sb = b, ra = a, rb = b;
+    int rk = key;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        rk = dpp_z<0x138>(rk);
+        ra = __int_as_float(dpp_z<0x138>(__float_as_int(ra)));
+        rb = __int_as_float(dpp_z<0x138>(__float_as_int(rb)));
+        const bool same = rk == key;
+        sa += same ? ra : 0.0f;
+        sb += same ? rb : 0.0f;
+    }
+    a = sa;
+    b = sb;
+}
+
+// Segmented inclusive scan over the lanes of a wave for segments of at
+// most maxrun lanes: up to 4 lanes by seg_linear; longer segments by
+// Hillis-Steele inside each row of 16 lanes (row_shr 1, 2, 4, 8) and the
+// row carries (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2
+// and 3).  The last lane of a segment ends with the segment's sums.
+__device__ __forceinline__ void seg_scan(float &a, float &b, int key, int maxrun)
+{
+    if (maxrun <= 2) {
+        seg_linear<1>(a, b, key);
+    } else if (maxrun == 3) {
+        seg_linear<2>(a, b, key);
+    } else if (maxrun == 4) {
+        seg_linear<3>(a, b, key);
+    } else {
+        seg_step<0x111>(a, b, key);
+        seg_step<0x112>(a, b, key);
+        seg_step<0x114>(a, b, key);
+        if (maxrun > 8) seg_step<0x118>(a, b, key);
+        seg_carry<0x142, 0xa>(a, b, key);
+        seg_carry<0x143, 0xc>(a, b, key);
+    }
 }
 
 // Forward projection: one wave per (strip, view); a strip is the set of
@@ -905,20 +961,20 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     if (blockIdx.x >= (unsigned)(per * 8)) return;
     const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (b >= nb) return;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = wave_id();
     const int s = b / nvg, a = (b - s * nvg) * kColWaves + wid;
     if (a >= ag.A) return;  // no block-wide barrier below
     const int nu = ag.nu, nv = ag.nv, L = nu + 2;
-    float *rm = col_rows + wid * 2 * L, *rn = rm + L;
-    for (int q = lane; q < 2 * L; q += 64) rm[q] = 0.0f;
+    // the wave's two detector rows, interleaved: (row k0, row k0 + 1)
+    float2 *rw = reinterpret_cast<float2 *>(col_rows) + wid * L;
+    for (int q = lane; q < L; q += 64) rw[q] = make_float2(0.0f, 0.0f);
     wave_lds_sync();
     const ViewCoef vc = ag.coef[a];
     const double c0 = vc.c[0], c1 = vc.c[1], c2 = vc.c[2], c3 = vc.c[3];
     // longest run of consecutive planes sharing floor(u): 1/|xc3| + 1
     const double ac3 = fabs(c3);
     const int maxrun = ac3 > 0.0 ? (int)fmin(64.0, floor(1.0 / ac3) + 1.0) : 64;
-    const int steps = maxrun <= 1 ? 0 : 32 - __clz(maxrun - 1);
-    const T *x = reinterpret_cast<const T *>(ag.x);
+    const auto x = gview(reinterpret_cast<const T *>(ag.x));
     const int nz = ag.Z, zs = ag.Zs;
     const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
     for (int j = j0; j < j1; j++) {
@@ -926,31 +982,36 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
         const float2 cw = ag.col_w[j];
         const float wm = cw.x, wn = cw.y;
         const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
-        const int64_t coff = (int64_t)(xy.x - ag.Xs) * ag.sx + (int64_t)(xy.y - ag.Ys) * ag.sy;
+        const auto xcol = x + ((int64_t)(xy.x - ag.Xs) * ag.sx + (int64_t)(xy.y - ag.Ys) * ag.sy);
         for (int zc = 0; zc < nz; zc += 64) {
             const int zi = zc + lane;
             const bool ok = zi < nz;
             float xv = 0.0f;
-            if (ok) xv = UNIT ? 1.0f : ldf<T>(x, coff + zi);
+            if (ok) xv = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)zi);
             // the reference's chain: fma(xc3, Z, fma(xc2, Y, fma(xc1, X, xc0))) - 1
             const double u = fma(c3, (double)(zs + zi), bxy) - 1.0;
             const double fl = floor(u);
-            const int key = ok ? (int)fl : 0x40000000 + lane;
+            const int key = ok ? (int)fl + kKeyBias : 0x40000000 + lane;
             // tap weights rounded from FP64 (no cancellation in 1 - du)
             const double dd = u - fl;
             float ea = (float)(1.0 - dd) * xv, eb = (float)dd * xv;
-            seg_scan(ea, eb, key, steps);
-            const int nkey = __shfl_down(key, 1, 64);
-            const bool lead = ok && (lane == 63 || nkey != key);
-            const int bi = key + 1;  // LDS slot of detector column key (column -1 -> slot 0)
+            seg_scan(ea, eb, key, maxrun);
+            // next lane's key (wave_shl 1; lane 63 reads 0)
+            const int nkey = dpp_z<0x130>(key);
+            const bool lead = ok && nkey != key;
+            const int bi = key - kKeyBias + 1;  // LDS slot of detector column floor(u) (column -1 -> slot 0)
             if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
-                rm[bi] += wm * ea;
-                rn[bi] += wn * ea;
+                float2 q = rw[bi];
+                q.x = fmaf(wm, ea, q.x);
+                q.y = fmaf(wn, ea, q.y);
+                rw[bi] = q;
             }
             wave_lds_sync();
             if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
-                rm[bi + 1] += wm * eb;
-                rn[bi + 1] += wn * eb;
+                float2 q = rw[bi + 1];
+                q.x = fmaf(wm, eb, q.x);
+                q.y = fmaf(wn, eb, q.y);
+                rw[bi + 1] = q;
             }
             wave_lds_sync();
         }
@@ -958,8 +1019,9 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     const int k0 = s - 1;
     const int64_t row = ((int64_t)a * nv + k0) * nu;
     for (int i = lane; i < nu; i += 64) {
-        if (k0 >= 0) ag.pm[row + i] = rm[i + 1];
-        if (k0 + 1 < nv) ag.pn[row + nu + i] = rn[i + 1];
+        const float2 q = rw[i + 1];
+        if (k0 >= 0) ag.pm[row + i] = q.x;
+        if (k0 + 1 < nv) ag.pn[row + nu + i] = q.y;
     }
 }
 
@@ -1022,7 +1084,7 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
     const int bz = b % ntz, b1 = (b / ntz) % nt1, b2 = b / (ntz * nt1);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = wave_id();
     const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kColBpWaves, Z0 = ag.Zs + bz * RW;
     const int p1 = P10 + lane, p2 = P20 + wid;
     const int e1 = min(s1 + n1 - 1, P10 + 63), e2 = min(s2 + n2 - 1, P20 + kColBpWaves - 1);
