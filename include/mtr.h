@@ -228,6 +228,43 @@ int mtr_calibrate_copy(mtr_ctx *ctx, size_t bytes, double *ms);
 /* Blocks until all queued work of the context is done. */
 int mtr_sync(mtr_ctx *ctx);
 
+/* ---- request server (continuous batching) ----------------------------
+ * No counterpart in the reference, which runs one reconstruction per MPI
+ * job (MultiAxis.cpp:101-230).  A server owns one device stream.  A
+ * request is a volume box and K tilt series (each with its own detector
+ * size and geometry) plus its solve parameters (M MACE steps of S SIRT
+ * iterations, lambda, rho); the server copies it on submission.  Requests
+ * are admitted in arrival order while their HBM footprint fits the budget
+ * and fewer than max_active are active; admission uploads the series and
+ * forms the BPT-sum initial model.  Each mtr_server_step runs one MACE
+ * step of every active request, the SIRT iterations of all their agents
+ * batched into shared launches, and retires the requests that finished
+ * (their w-bar is kept on the host and their device memory freed).  A
+ * request's result does not depend on what else shares its launches.
+ * Times are seconds since mtr_server_create. */
+typedef struct mtr_server mtr_server;
+int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out);
+int mtr_server_destroy(mtr_server *srv);
+/* Device bytes a request of this shape holds while active (upper bound). */
+int mtr_server_footprint(const mtr_server *srv, const mtr_volume_size *vol, int K,
+                         const mtr_projection_size *prj, size_t *bytes);
+/* MTR_ERR_OOM when the request alone exceeds the budget. */
+int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const mtr_projection_size *prj,
+                      const double *const *xc, const double *const *yc, const float *const *p, int M,
+                      int S, float lambda, float rho, int64_t *request_id);
+/* Admission, one MACE step of the active requests, retirement; returns
+ * after the step's device work has finished. */
+int mtr_server_step(mtr_server *srv, int *completed);
+int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *done, size_t *hbm_in_use);
+/* w-bar of a finished request ([Z][Y][X]); times[3] (nullable) receives
+ * submission, admission and completion times.  MTR_ERR_STATE while the
+ * request is pending or active. */
+int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, double *times);
+int mtr_server_now(const mtr_server *srv, double *seconds);
+/* Per-kernel timing of the batched launches (MTR_K_*), as for a context. */
+int mtr_server_timing(mtr_server *srv, int on);
+int mtr_server_timing_get(mtr_server *srv, int kernel, int64_t *launches, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -282,6 +282,106 @@ class Context:
         return xo
 
 
+class Server:
+    """Continuous-batching request server on one device (mtr_server_*).
+
+    submit() copies a request (a synth.Problem, or its parts) into the
+    server; step() admits pending requests that fit the HBM budget, runs one
+    MACE step of every active request with their SIRT iterations batched
+    into shared launches, and retires the finished ones."""
+
+    def __init__(self, device: int = 0, precision: int = PREC_FP32, hbm_budget: int = 200 << 30,
+                 max_active: int = 64):
+        self._h = _ct.c_void_p()
+        _check(lib.mtr_server_create(device, precision, hbm_budget, max_active, _ct.byref(self._h)),
+               "mtr_server_create")
+        self.shapes: dict[int, tuple[int, int, int]] = {}
+
+    def close(self) -> None:
+        if self._h:
+            lib.mtr_server_destroy(self._h)
+            self._h = _ct.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arrays(prjs):
+        return (ProjectionSize * len(prjs))(*[ProjectionSize(p.X, p.Y, p.AngN) for p in prjs])
+
+    def footprint(self, vol: VolumeSize, prjs) -> int:
+        b = _ct.c_size_t()
+        _check(lib.mtr_server_footprint(self._h, _ct.byref(vol), len(prjs), self._arrays(prjs), _ct.byref(b)),
+               "server_footprint")
+        return b.value
+
+    def submit(self, vol: VolumeSize, prjs, xcs, ycs, ps, M: int, S: int, lam: float, rho: float) -> int:
+        K = len(prjs)
+        xcs = [_f64(x) for x in xcs]
+        ycs = [_f64(y) for y in ycs]
+        ps = [_f32(p, (pr.AngN, pr.Y, pr.X)) for p, pr in zip(ps, prjs)]
+        PD = _ct.POINTER(_ct.c_double)
+        PF = _ct.POINTER(_ct.c_float)
+        xa = (PD * K)(*[_dp(x) for x in xcs])
+        ya = (PD * K)(*[_dp(y) for y in ycs])
+        pa = (PF * K)(*[_fp(p) for p in ps])
+        rid = _ct.c_int64()
+        _check(lib.mtr_server_submit(self._h, _ct.byref(vol), K, self._arrays(prjs), xa, ya, pa, M, S, lam, rho,
+                                     _ct.byref(rid)), "server_submit")
+        self.shapes[rid.value] = (vol.Z, vol.Y, vol.X)
+        return rid.value
+
+    def submit_problem(self, pb) -> int:
+        return self.submit(pb.vol, pb.prjs, pb.xcs, pb.ycs, pb.ps, pb.M, pb.S, pb.lam, pb.rho)
+
+    def step(self) -> int:
+        n = _ct.c_int()
+        _check(lib.mtr_server_step(self._h, _ct.byref(n)), "server_step")
+        return n.value
+
+    def status(self) -> dict:
+        p, a, d = _ct.c_int(), _ct.c_int(), _ct.c_int()
+        u = _ct.c_size_t()
+        _check(lib.mtr_server_status(self._h, _ct.byref(p), _ct.byref(a), _ct.byref(d), _ct.byref(u)),
+               "server_status")
+        return {"pending": p.value, "active": a.value, "done": d.value, "hbm_in_use": u.value}
+
+    def result(self, rid: int):
+        """(w-bar, (submitted, admitted, completed) seconds)."""
+        out = _np.empty(self.shapes[rid], _np.float32)
+        t = (_ct.c_double * 3)()
+        _check(lib.mtr_server_result(self._h, rid, _fp(out), t), "server_result")
+        return out, tuple(t)
+
+    def times(self, rid: int):
+        t = (_ct.c_double * 3)()
+        lib.mtr_server_result(self._h, rid, None, t)
+        return tuple(t)
+
+    def now(self) -> float:
+        t = _ct.c_double()
+        _check(lib.mtr_server_now(self._h, _ct.byref(t)), "server_now")
+        return t.value
+
+    def timing(self, on: bool) -> None:
+        _check(lib.mtr_server_timing(self._h, int(on)), "server_timing")
+
+    def timing_get(self, kernel: int) -> tuple[int, float]:
+        n = _ct.c_int64()
+        ms = _ct.c_double()
+        _check(lib.mtr_server_timing_get(self._h, kernel, _ct.byref(n), _ct.byref(ms)), "server_timing_get")
+        return n.value, ms.value
+
+
 def gpu_available() -> bool:
     """True when the library can open a context on device 0."""
     h = _ct.c_void_p()

@@ -80,6 +80,18 @@ _PROTOS = {
     "mtr_timing_get": (_i, [_v, _i, ct.POINTER(_i64), _pd]),
     "mtr_sync": (_i, [_v]),
     "mtr_calibrate_copy": (_i, [_v, _sz, _pd]),
+    # request server
+    "mtr_server_create": (_i, [_i, _i, _sz, _i, ct.POINTER(_v)]),
+    "mtr_server_destroy": (_i, [_v]),
+    "mtr_server_footprint": (_i, [_v, _pvol, _i, _pprj, ct.POINTER(_sz)]),
+    "mtr_server_submit": (_i, [_v, _pvol, _i, _pprj, ct.POINTER(_pd), ct.POINTER(_pd), ct.POINTER(_pf), _i, _i, _f,
+                               _f, ct.POINTER(_i64)]),
+    "mtr_server_step": (_i, [_v, _pi]),
+    "mtr_server_status": (_i, [_v, _pi, _pi, _pi, ct.POINTER(_sz)]),
+    "mtr_server_result": (_i, [_v, _i64, _pf, _pd]),
+    "mtr_server_now": (_i, [_v, _pd]),
+    "mtr_server_timing": (_i, [_v, _i]),
+    "mtr_server_timing_get": (_i, [_v, _i, ct.POINTER(_i64), _pd]),
     # file boundary (mtr_io.h)
     "mtr_mrc_read_header": (_i, [ct.c_char_p, _v]),
     "mtr_mrc_read_data": (_i, [ct.c_char_p, _pf, _i64]),

@@ -3,7 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <deque>
+#include <map>
+#include <memory>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -335,6 +339,7 @@ struct Agent {
 struct mtr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false: the stream belongs to a server
     int precision = MTR_PREC_FP32;
     mtr_volume_size vol{};
     bool vol_set = false;
@@ -500,80 +505,106 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
     return MTR_OK;
 }
 
-// One fused FP+residual launch per maximal run of consecutive agents that
-// share a forward-projection kernel.
-int fp_group(mtr_ctx *ctx, const std::vector<int> &ids)
+// A batch of resident agents, possibly of several contexts (volumes):
+// entry i is described by d_tab[i] on the device.  Launches go to the
+// engine context's stream; consecutive entries sharing a kernel share one
+// launch (blockIdx.y selects the entry, each entry sizes its own grid).
+struct BatchEntry {
+    mtr_ctx *owner;
+    Agent *ag;
+};
+struct Batch {
+    std::vector<BatchEntry> e;
+    AgentDev *d_tab = nullptr;
+};
+
+int fp_kind_key(const Agent &a) { return a.col.on ? 100 + a.col.rw : (a.fp_nc ? 10 + a.fp_nc : (a.lanes_k ? 1 : 0)); }
+int bp_kind_key(const Agent &a)
 {
-    size_t s = 0;
-    while (s < ids.size()) {
-        const Agent &a0 = ctx->agents[ids[s]];
-        size_t e = s + 1;
-        while (e < ids.size() && ids[e] == ids[e - 1] + 1 && ctx->agents[ids[e]].lanes_k == a0.lanes_k &&
-               ctx->agents[ids[e]].fp_nc == a0.fp_nc && ctx->agents[ids[e]].col.on == a0.col.on)
-            e++;
-        if (a0.col.on) {
-            int mb = 0, mr = 0, nu = 0;
-            for (size_t q = s; q < e; q++) {
-                Agent &ag = ctx->agents[ids[q]];
-                const AgentDev d = make_dev(ctx, ag);
+    return a.col.on ? 100 + a.col.rw : (a.bpm.sep ? 10 + 4 * a.bpm.W + (a.bpm.rw == 32 ? 1 : (a.bpm.rw == 16 ? 2 : 3)) : 0);
+}
+
+// FP + residual of entries [first, first + count).
+int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
+{
+    int s = first;
+    const int end = first + count;
+    while (s < end) {
+        const Agent &a0 = *B.e[s].ag;
+        int e = s + 1;
+        while (e < end && fp_kind_key(*B.e[e].ag) == fp_kind_key(a0)) e++;
+        int mb = 0, mr = 0, nu = 0;
+        for (int q = s; q < e; q++) {
+            Agent &ag = *B.e[q].ag;
+            const AgentDev d = make_dev(B.e[q].owner, ag);
+            if (a0.col.on) {
                 ag.npartial = fp_col_res_blocks(d);
                 mb = std::max(mb, fp_col_blocks(d));
                 mr = std::max(mr, ag.npartial);
                 nu = std::max(nu, ag.prj.X);
+            } else {
+                ag.npartial = ag.fp_nc ? fp_stream_blocks(d) : ag.nfp;
+                mb = std::max(mb, ag.npartial);
             }
-            Timed t(ctx, MTR_K_FP);
-            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, nu, false, ctx->stream));
-            HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mr, 0, nullptr,
-                                      ctx->stream));
-            s = e;
-            continue;
         }
-        int mb = 0;
-        for (size_t q = s; q < e; q++) {
-            Agent &ag = ctx->agents[ids[q]];
-            const AgentDev d = make_dev(ctx, ag);
-            ag.npartial = ag.fp_nc ? fp_stream_blocks(d) : ag.nfp;
-            mb = std::max(mb, ag.npartial);
+        Timed t(eng, MTR_K_FP);
+        if (a0.col.on) {
+            HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
+            HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
+        } else if (a0.fp_nc) {
+            HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, eng->stream));
+        } else {
+            HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, eng->stream));
         }
-        Timed t(ctx, MTR_K_FP);
-        if (a0.fp_nc)
-            HIP_TRY(launch_fp_stream(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, a0.fp_nc, ctx->stream));
-        else
-            HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + ids[s], (int)(e - s), mb, 0, a0.lanes_k, nullptr, nullptr,
-                              ctx->stream));
         s = e;
     }
     return MTR_OK;
 }
 
-int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
+// BP (bpt: initial model) of entries [first, first + count).
+int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float lambda)
 {
-    // launch each maximal run of agents sharing a back-projection mode
     int s = first;
     const int end = first + count;
     while (s < end) {
-        const BpMode &m = ctx->agents[s].bpm;
+        const Agent &a0 = *B.e[s].ag;
         int e = s + 1;
-        const ColMode &cm = ctx->agents[s].col;
-        while (e < end && ctx->agents[e].bpm.sep == m.sep && ctx->agents[e].bpm.W == m.W &&
-               ctx->agents[e].col.on == cm.on && ctx->agents[e].col.rw == cm.rw)
-            e++;
-        const AgentDev d = make_dev(ctx, ctx->agents[s]);
-        Timed t(ctx, bpt ? MTR_K_BPT : MTR_K_BP);
+        while (e < end && bp_kind_key(*B.e[e].ag) == bp_kind_key(a0)) e++;
+        const BpMode &m = a0.bpm;
+        const ColMode &cm = a0.col;
+        int mb = 0;
+        for (int q = s; q < e; q++) {
+            const AgentDev d = make_dev(B.e[q].owner, *B.e[q].ag);
+            mb = std::max(mb, cm.on ? bp_col_blocks(d, cm.rw)
+                                    : (m.sep ? bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw) : bp_blocks(d, eng->bp_tz)));
+        }
+        Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP);
         if (cm.on) {
-            int mb = 0;
-            for (int q = s; q < e; q++) mb = std::max(mb, bp_col_blocks(make_dev(ctx, ctx->agents[q]), cm.rw));
-            HIP_TRY(launch_bp_col(ctx->precision, ctx->d_ags + s, e - s, mb, bpt, lambda, cm.rw, ctx->stream));
+            HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, cm.rw, eng->stream));
         } else if (m.sep) {
-            HIP_TRY(launch_bp_sep(ctx->precision, ctx->d_ags + s, e - s, bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw), bpt,
-                                  lambda, m.W, m.P1, m.P2, m.rw, ctx->stream));
+            HIP_TRY(launch_bp_sep(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, m.W, m.P1, m.P2, m.rw,
+                                  eng->stream));
         } else {
-            HIP_TRY(launch_bp(ctx->precision, ctx->d_ags + s, e - s, bp_blocks(d, ctx->bp_tz), ctx->bp_tz, bpt, lambda,
-                              nullptr, ctx->lds_cap, ctx->stream));
+            HIP_TRY(launch_bp(eng->precision, B.d_tab + s, e - s, mb, eng->bp_tz, bpt, lambda, nullptr, eng->lds_cap,
+                              eng->stream));
         }
         s = e;
     }
     return MTR_OK;
+}
+
+// The agents [first, first + count) of one context as a batch.
+Batch ctx_batch(mtr_ctx *ctx, int first, int count)
+{
+    Batch B;
+    for (int a = first; a < first + count; a++) B.e.push_back({ctx, &ctx->agents[a]});
+    B.d_tab = ctx->d_ags + first;
+    return B;
+}
+
+int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
+{
+    return bp_batch(ctx, ctx_batch(ctx, first, count), 0, count, bpt, lambda);
 }
 
 int ensure_plain_scratch(mtr_ctx *ctx, int64_t P)
@@ -642,16 +673,18 @@ int sirt_plain(mtr_ctx *ctx, int a, int iters, float lambda)
 int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda)
 {
     for (int a : ids) MTR_TRY(ensure_normaliser(ctx, a));
-    // agents of a context are contiguous from 0; BP launches the whole run.
+    // one batch per contiguous run of agents
+    std::vector<Batch> runs;
+    size_t s = 0;
+    while (s < ids.size()) {
+        size_t e = s + 1;
+        while (e < ids.size() && ids[e] == ids[e - 1] + 1) e++;
+        runs.push_back(ctx_batch(ctx, ids[s], (int)(e - s)));
+        s = e;
+    }
     for (int it = 0; it < iters; it++) {
-        MTR_TRY(fp_group(ctx, ids));
-        size_t s = 0;
-        while (s < ids.size()) {
-            size_t e = s + 1;
-            while (e < ids.size() && ids[e] == ids[e - 1] + 1) e++;
-            MTR_TRY(bp_group(ctx, ids[s], (int)(e - s), false, lambda));
-            s = e;
-        }
+        for (const Batch &B : runs) MTR_TRY(fp_batch(ctx, B, 0, (int)B.e.size()));
+        for (const Batch &B : runs) MTR_TRY(bp_batch(ctx, B, 0, (int)B.e.size(), false, lambda));
     }
     return MTR_OK;
 }
@@ -1033,7 +1066,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         (void)hipEventDestroy(pd.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MTR_OK;
 }
@@ -1533,6 +1566,353 @@ int mtr_sync(mtr_ctx *ctx)
     MTR_TRY(set_device(ctx));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return collect_timing(ctx);
+}
+
+}  // extern "C"
+
+/* ---- request server ------------------------------------------------- */
+
+namespace {
+
+enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3 };
+
+struct ServerReq {
+    int64_t id = 0;
+    int state = REQ_PENDING;
+    mtr_volume_size vol{};
+    std::vector<mtr_projection_size> prj;
+    std::vector<std::vector<double>> xc, yc;
+    std::vector<std::vector<float>> p;  // host copies until admission
+    int M = 0, S = 0;
+    float lambda = 0.f, rho = 0.f;
+    int step = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    size_t bytes = 0;
+    mtr_ctx *ctx = nullptr;
+    std::vector<float> result;
+    std::string error;
+};
+
+}  // namespace
+
+struct mtr_server {
+    int device = 0;
+    int precision = MTR_PREC_FP32;
+    size_t budget = 0;
+    int max_active = 0;
+    mtr_ctx *eng = nullptr;  // engine context: the stream every request shares
+    size_t used = 0;
+    int64_t next_id = 1;
+    int n_done = 0;
+    std::deque<ServerReq *> pending;
+    std::vector<ServerReq *> active;
+    std::map<int64_t, std::unique_ptr<ServerReq>> reqs;
+    AgentDev *d_tab = nullptr;
+    int tab_cap = 0;
+    std::vector<AgentDev> h_tab;
+    std::chrono::steady_clock::time_point t0;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+size_t request_bytes(int precision, const mtr_volume_size &vol, int K, const mtr_projection_size *prj)
+{
+    const size_t es = elem_size(precision);
+    const size_t N = (size_t)vol.X * vol.Y * vol.Z;
+    const size_t cols = (size_t)vol.X * vol.Y;
+    size_t b = es * N + sizeof(AgentDev) * kMaxAgents + 8 * 8193;  // w-bar, tables, consensus partials
+    for (int a = 0; a < K; a++) {
+        const size_t P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
+        const size_t nfp = (size_t)((prj[a].X + 63) / 64 + 16) * ((prj[a].Y + 3) / 4 + 16) * prj[a].AngN;
+        b += (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj[a].AngN + 2 * 4 * P + 4 * P + es * P +
+             2 * 4 * P + 3 * es * N + 12 * cols + 4 * (size_t)(prj[a].Y + 2) + 8 * nfp;
+    }
+    return b;
+}
+
+void release(mtr_server *srv, ServerReq *r)
+{
+    if (r->ctx) {
+        mtr_ctx_destroy(r->ctx);
+        r->ctx = nullptr;
+    }
+    srv->used -= std::min(srv->used, r->bytes);
+}
+
+// Creates the request's context on the server's stream, uploads its tilt
+// series and forms the BPT-sum initial model (MultiAxis.cpp:163-200).
+int admit(mtr_server *srv, ServerReq *r)
+{
+    mtr_ctx *c = new mtr_ctx();
+    c->device = srv->device;
+    c->stream = srv->eng->stream;
+    c->own_stream = false;
+    c->precision = srv->precision;
+    r->ctx = c;
+    if (hipMalloc(&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&c->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
+        hipMalloc(&c->d_cons_scalar, sizeof(double)) != hipSuccess)
+        return fail(MTR_ERR_OOM, "request tables");
+    MTR_TRY(mtr_ctx_set_volume(c, &r->vol));
+    for (size_t a = 0; a < r->prj.size(); a++) {
+        int id = 0;
+        MTR_TRY(mtr_agent_add(c, &r->prj[a], r->xc[a].data(), r->yc[a].data(), r->p[a].data(), &id));
+        std::vector<float>().swap(r->p[a]);  // resident now
+    }
+    for (int a = 0; a < (int)c->agents.size(); a++) MTR_TRY(ensure_normaliser(c, a));
+    MTR_TRY(mtr_mace_init(c));
+    return MTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out)
+{
+    if (!out) return fail(MTR_ERR_ARG, "null out");
+    *out = nullptr;
+    if (precision != MTR_PREC_FP32 && precision != MTR_PREC_BF16 && precision != MTR_PREC_FP16)
+        return fail(MTR_ERR_ARG, "unknown precision %d", precision);
+    if (max_active <= 0 || hbm_budget == 0) return fail(MTR_ERR_ARG, "empty server");
+    mtr_ctx *eng = nullptr;
+    MTR_TRY(mtr_ctx_create(device, &eng));
+    eng->precision = precision;
+    mtr_server *srv = new mtr_server();
+    srv->device = device;
+    srv->precision = precision;
+    srv->budget = hbm_budget;
+    srv->max_active = max_active;
+    srv->eng = eng;
+    srv->t0 = std::chrono::steady_clock::now();
+    *out = srv;
+    return MTR_OK;
+}
+
+int mtr_server_destroy(mtr_server *srv)
+{
+    if (!srv) return MTR_OK;
+    (void)hipSetDevice(srv->device);
+    (void)hipStreamSynchronize(srv->eng->stream);
+    for (auto &kv : srv->reqs) release(srv, kv.second.get());
+    if (srv->d_tab) (void)hipFree(srv->d_tab);
+    mtr_ctx_destroy(srv->eng);
+    delete srv;
+    return MTR_OK;
+}
+
+int mtr_server_footprint(const mtr_server *srv, const mtr_volume_size *vol, int K, const mtr_projection_size *prj,
+                         size_t *bytes)
+{
+    if (!srv || !vol || !prj || !bytes) return fail(MTR_ERR_ARG, "null argument");
+    if (!vol_valid(vol)) return fail(MTR_ERR_SHAPE, "invalid volume box");
+    if (K <= 0 || K > kMaxAgents) return fail(MTR_ERR_RANGE, "agent count %d outside 1..%d", K, kMaxAgents);
+    for (int a = 0; a < K; a++)
+        if (!prj_valid(&prj[a])) return fail(MTR_ERR_SHAPE, "invalid projection size of agent %d", a);
+    *bytes = request_bytes(srv->precision, *vol, K, prj);
+    return MTR_OK;
+}
+
+int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const mtr_projection_size *prj,
+                      const double *const *xc, const double *const *yc, const float *const *p, int M, int S,
+                      float lambda, float rho, int64_t *request_id)
+{
+    if (!srv || !xc || !yc || !p) return fail(MTR_ERR_ARG, "null argument");
+    size_t bytes = 0;
+    MTR_TRY(mtr_server_footprint(srv, vol, K, prj, &bytes));
+    if (M < 0 || S < 0) return fail(MTR_ERR_ARG, "negative iteration count");
+    if (bytes > srv->budget)
+        return fail(MTR_ERR_OOM, "request needs %zu bytes, the server's budget is %zu", bytes, srv->budget);
+    auto r = std::make_unique<ServerReq>();
+    r->id = srv->next_id++;
+    r->vol = *vol;
+    r->prj.assign(prj, prj + K);
+    for (int a = 0; a < K; a++) {
+        if (!xc[a] || !yc[a] || !p[a]) return fail(MTR_ERR_ARG, "null series %d", a);
+        const size_t A = (size_t)prj[a].AngN, P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
+        r->xc.emplace_back(xc[a], xc[a] + 4 * A);
+        r->yc.emplace_back(yc[a], yc[a] + 4 * A);
+        r->p.emplace_back(p[a], p[a] + P);
+    }
+    r->M = M;
+    r->S = S;
+    r->lambda = lambda;
+    r->rho = rho;
+    r->bytes = bytes;
+    r->t_submit = srv->now();
+    if (request_id) *request_id = r->id;
+    srv->pending.push_back(r.get());
+    srv->reqs[r->id] = std::move(r);
+    return MTR_OK;
+}
+
+// One scheduling tick: admit pending requests in arrival order while the
+// HBM budget and the slot count allow, run one MACE step of every active
+// request with their SIRT iterations batched into shared launches, then
+// retire the requests that finished.
+int mtr_server_step(mtr_server *srv, int *completed)
+{
+    if (!srv) return fail(MTR_ERR_ARG, "null server");
+    if (completed) *completed = 0;
+    MTR_TRY(set_device(srv->eng));
+    int done = 0;
+    while (!srv->pending.empty() && (int)srv->active.size() < srv->max_active) {
+        ServerReq *r = srv->pending.front();
+        if (srv->used + r->bytes > srv->budget) break;
+        srv->pending.pop_front();
+        srv->used += r->bytes;
+        r->t_admit = srv->now();
+        const int rc = admit(srv, r);
+        if (rc != MTR_OK) {
+            r->error = g_err;
+            r->state = REQ_FAILED;
+            release(srv, r);
+            continue;
+        }
+        r->state = REQ_ACTIVE;
+        srv->active.push_back(r);
+    }
+    // batch: every agent of every active request with steps left, most SIRT
+    // iterations first, then by relaxation and kernel
+    Batch B;
+    std::vector<const ServerReq *> owner;
+    for (ServerReq *r : srv->active)
+        if (r->step < r->M)
+            for (auto &ag : r->ctx->agents) {
+                B.e.push_back({r->ctx, &ag});
+                owner.push_back(r);
+            }
+    std::vector<int> order(B.e.size());
+    for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (owner[x]->S != owner[y]->S) return owner[x]->S > owner[y]->S;
+        if (owner[x]->lambda != owner[y]->lambda) return owner[x]->lambda < owner[y]->lambda;
+        const int fx = fp_kind_key(*B.e[x].ag), fy = fp_kind_key(*B.e[y].ag);
+        if (fx != fy) return fx < fy;
+        return bp_kind_key(*B.e[x].ag) < bp_kind_key(*B.e[y].ag);
+    });
+    {
+        Batch sorted;
+        std::vector<const ServerReq *> own2;
+        for (int q : order) {
+            sorted.e.push_back(B.e[q]);
+            own2.push_back(owner[q]);
+        }
+        B = std::move(sorted);
+        owner = std::move(own2);
+    }
+    const int n = (int)B.e.size();
+    if (n > 0) {
+        if (n > srv->tab_cap) {
+            if (srv->d_tab) (void)hipFree(srv->d_tab);
+            srv->d_tab = nullptr;
+            srv->tab_cap = 0;
+            HIP_TRY(hipMalloc(&srv->d_tab, sizeof(AgentDev) * n));
+            srv->tab_cap = n;
+        }
+        srv->h_tab.resize(n);
+        for (int q = 0; q < n; q++) srv->h_tab[q] = make_dev(B.e[q].owner, *B.e[q].ag);
+        HIP_TRY(hipMemcpyAsync(srv->d_tab, srv->h_tab.data(), sizeof(AgentDev) * n, hipMemcpyHostToDevice,
+                               srv->eng->stream));
+        B.d_tab = srv->d_tab;
+        const int maxS = owner[0]->S;
+        for (int it = 0; it < maxS; it++) {
+            int m = 0;
+            while (m < n && owner[m]->S > it) m++;
+            MTR_TRY(fp_batch(srv->eng, B, 0, m));
+            int s0 = 0;
+            while (s0 < m) {
+                int e0 = s0 + 1;
+                while (e0 < m && owner[e0]->lambda == owner[s0]->lambda) e0++;
+                MTR_TRY(bp_batch(srv->eng, B, s0, e0 - s0, false, owner[s0]->lambda));
+                s0 = e0;
+            }
+        }
+    }
+    // consensus of every active request (MultiAxis.cpp:40-71)
+    for (ServerReq *r : srv->active) {
+        if (r->step >= r->M) continue;
+        mtr_ctx *c = r->ctx;
+        std::vector<void *> w, x;
+        for (auto &ag : c->agents) {
+            MTR_TRY(to_canon(c, ag));
+            w.push_back(ag.d_w);
+            x.push_back(work_canon(ag));
+        }
+        {
+            Timed t(srv->eng, MTR_K_CONSENSUS);
+            HIP_TRY(launch_consensus(c->precision, (int)c->agents.size(), c->N, r->rho, 1, w.data(), x.data(),
+                                     c->d_wbar, c->d_cons_partial, consensus_blocks(c->N), c->stream));
+        }
+        for (auto &ag : c->agents) MTR_TRY(from_canon(c, ag));
+        r->step++;
+    }
+    HIP_TRY(hipStreamSynchronize(srv->eng->stream));
+    std::vector<ServerReq *> still;
+    for (ServerReq *r : srv->active) {
+        if (r->step < r->M) {
+            still.push_back(r);
+            continue;
+        }
+        r->result.resize((size_t)r->ctx->N);
+        const int rc = copy_volume_out(r->ctx, r->ctx->d_wbar, r->result.data());
+        r->t_done = srv->now();
+        if (rc != MTR_OK) {
+            r->error = g_err;
+            r->state = REQ_FAILED;
+        } else {
+            r->state = REQ_DONE;
+        }
+        release(srv, r);
+        done++;
+        srv->n_done++;
+    }
+    srv->active.swap(still);
+    MTR_TRY(collect_timing(srv->eng));
+    if (completed) *completed = done;
+    return MTR_OK;
+}
+
+int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *done, size_t *hbm_in_use)
+{
+    if (!srv) return fail(MTR_ERR_ARG, "null server");
+    if (pending) *pending = (int)srv->pending.size();
+    if (active) *active = (int)srv->active.size();
+    if (done) *done = srv->n_done;
+    if (hbm_in_use) *hbm_in_use = srv->used;
+    return MTR_OK;
+}
+
+int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, double *times)
+{
+    if (!srv) return fail(MTR_ERR_ARG, "null server");
+    auto it = srv->reqs.find(request_id);
+    if (it == srv->reqs.end()) return fail(MTR_ERR_RANGE, "unknown request %lld", (long long)request_id);
+    ServerReq *r = it->second.get();
+    if (times) {
+        times[0] = r->t_submit;
+        times[1] = r->t_admit;
+        times[2] = r->t_done;
+    }
+    if (r->state == REQ_FAILED) return fail(MTR_ERR_HIP, "request %lld failed: %s", (long long)request_id, r->error.c_str());
+    if (r->state != REQ_DONE) return fail(MTR_ERR_STATE, "request %lld not finished", (long long)request_id);
+    if (wbar_host) std::memcpy(wbar_host, r->result.data(), sizeof(float) * r->result.size());
+    return MTR_OK;
+}
+
+int mtr_server_now(const mtr_server *srv, double *seconds)
+{
+    if (!srv || !seconds) return fail(MTR_ERR_ARG, "null argument");
+    *seconds = srv->now();
+    return MTR_OK;
+}
+
+int mtr_server_timing(mtr_server *srv, int on) { return srv ? mtr_timing_enable(srv->eng, on) : fail(MTR_ERR_ARG, "null server"); }
+
+int mtr_server_timing_get(mtr_server *srv, int kernel, int64_t *launches, double *ms)
+{
+    return srv ? mtr_timing_get(srv->eng, kernel, launches, ms) : fail(MTR_ERR_ARG, "null server");
 }
 
 }  // extern "C"

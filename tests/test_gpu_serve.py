@@ -1,0 +1,126 @@
+"""Request server (continuous batching) on the GPU.
+
+A request's result must not depend on what shares its launches: the same
+request is solved alone, in a mix of requests of other sizes, axis counts
+and iteration counts that arrive while it is active, and by a standalone
+context.  Batching only puts several agents into one launch; each agent's
+blocks, sums and reductions are its own, so the three results are compared
+bit for bit (a stricter bar than the FP32 tolerance, 1e-5 relative L2,
+which the oracle comparison uses).  Admission follows arrival order within
+the HBM budget and the slot count.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+SOLVE_TOL = 1e-5
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def req(mtr, X, Y, Z, phis, seed, M=3, S=2):
+    return mtr.synth.make_problem("srv", X, Y, Z, 6, phis, seed=seed, amin=2, amax=6,
+                                  theta=np.linspace(-60, 60, 9), M=M, S=S, lam=0.5, rho=0.5)
+
+
+def drain(srv, n):
+    steps = 0
+    while srv.status()["done"] < n:
+        srv.step()
+        steps += 1
+        assert steps < 1000
+    return steps
+
+
+def test_result_is_the_same_alone_in_a_mix_and_standalone(mtr, orc):
+    pbs = {
+        "a": req(mtr, 40, 36, 12, [0.0, 90.0], 41),
+        "b": req(mtr, 48, 40, 16, [0.0, 60.0, 120.0], 42),  # the probe
+        "c": req(mtr, 32, 32, 10, [0.0], 43, M=2),
+        "d": req(mtr, 36, 44, 14, [0.0, 45.0, 90.0, 135.0], 44, M=4, S=3),
+    }
+    say("serve alone")
+    with mtr.Server(0) as srv:
+        rb = srv.submit_problem(pbs["b"])
+        drain(srv, 1)
+        alone, _ = srv.result(rb)
+    say("serve mix")
+    with mtr.Server(0) as srv:
+        ids = {"a": srv.submit_problem(pbs["a"])}
+        srv.step()
+        ids["b"] = srv.submit_problem(pbs["b"])
+        ids["c"] = srv.submit_problem(pbs["c"])
+        srv.step()
+        assert srv.status()["active"] >= 2  # b runs alongside a
+        ids["d"] = srv.submit_problem(pbs["d"])
+        drain(srv, 4)
+        mix = {k: srv.result(r)[0] for k, r in ids.items()}
+    assert np.array_equal(alone, mix["b"])
+    say("standalone context")
+    pb = pbs["b"]
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        for a in range(pb.K):
+            c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+        solo = c.mace_solve(pb.M, pb.S, pb.lam, pb.rho)
+    assert np.array_equal(solo, alone)
+    for k, pb in pbs.items():
+        ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, pb.M, pb.S, pb.lam, pb.rho)
+        err = orc.rel_l2(mix[k], ref)
+        say(f"request {k}: rel_l2 vs oracle {err:.2e}")
+        assert err < SOLVE_TOL
+
+
+def test_admission_follows_hbm_budget_in_arrival_order(mtr):
+    pbs = [req(mtr, 40, 40, 12, [0.0, 90.0], 50 + q, M=2) for q in range(3)]
+    with mtr.Server(0) as probe:
+        need = probe.footprint(pbs[0].vol, pbs[0].prjs)
+    budget = int(need * 1.5)  # room for one request at a time
+    say("budget", need, budget)
+    with mtr.Server(0, hbm_budget=budget, max_active=8) as srv:
+        ids = [srv.submit_problem(p) for p in pbs]
+        assert srv.status()["pending"] == 3
+        srv.step()
+        st = srv.status()
+        assert st["active"] == 1 and st["pending"] == 2 and 0 < st["hbm_in_use"] <= budget
+        drain(srv, 3)
+        t = [srv.times(r) for r in ids]
+        assert srv.status()["hbm_in_use"] == 0
+    # admitted in arrival order, each after its predecessor retired
+    for q in range(1, 3):
+        assert t[q][1] >= t[q - 1][2]
+
+
+def test_slot_limit_and_oversized_request(mtr):
+    pbs = [req(mtr, 24, 24, 8, [0.0], 60 + q, M=3) for q in range(4)]
+    with mtr.Server(0, max_active=2) as srv:
+        for p in pbs:
+            srv.submit_problem(p)
+        srv.step()
+        assert srv.status()["active"] == 2 and srv.status()["pending"] == 2
+        drain(srv, 4)
+    with mtr.Server(0) as probe:
+        need = probe.footprint(pbs[0].vol, pbs[0].prjs)
+    with mtr.Server(0, hbm_budget=need // 2) as srv:
+        with pytest.raises(mtr.MtrError) as ei:
+            srv.submit_problem(pbs[0])
+        assert ei.value.code == -3
+
+
+def test_zero_step_request_returns_the_initial_model(mtr):
+    pb = req(mtr, 30, 26, 9, [0.0, 45.0], 70, M=0)
+    with mtr.Server(0) as srv:
+        r = srv.submit_problem(pb)
+        drain(srv, 1)
+        got, _ = srv.result(r)
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        for a in range(pb.K):
+            c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+        ref = c.mace_solve(0, pb.S, pb.lam, pb.rho)
+    assert np.array_equal(got, ref)
