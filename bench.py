@@ -69,6 +69,73 @@ def cpu_baseline(pb, threads: int):
     }, (x, xs)
 
 
+def serving(mtr, args):
+    """Replays the seeded request stream against the continuous-batching
+    server: requests are submitted when the server clock passes their
+    Poisson arrival time; latency = completion - arrival."""
+    t0 = time.perf_counter()
+    stream = mtr.synth.serving_stream(n=args.serve_requests, threads=16)
+    gen_s = time.perf_counter() - t0
+    log(f"serving stream: {len(stream)} requests generated in {gen_s:.1f}s, last arrival {stream[-1][0]:.2f}s")
+    srv = mtr.Server(0, mtr.PREC_FP32, hbm_budget=int(args.hbm_budget_gb * (1 << 30)), max_active=args.max_active)
+    # warm-up request (first launches of every kernel), not part of the stream
+    warm = mtr.synth.make_problem("warm", 64, 64, 16, 4, [0.0, 45.0, 90.0], seed=9, M=1, S=1, lam=0.5, rho=0.5)
+    srv.submit_problem(warm)
+    while srv.status()["done"] < 1:
+        srv.step()
+    n = len(stream)
+    ids = []
+    q = 0
+    steps = 0
+    busy = 0.0
+    active_sum = 0
+    start = srv.now()
+    while srv.status()["done"] < n + 1:
+        now = srv.now() - start
+        while q < n and stream[q][0] <= now:
+            ids.append(srv.submit_problem(stream[q][1]))
+            stream[q] = (stream[q][0], None)  # the server holds its own copy
+            q += 1
+        st = srv.status()
+        if st["active"] + st["pending"] > 0:
+            t1 = srv.now()
+            srv.step()
+            busy += srv.now() - t1
+            steps += 1
+            active_sum += srv.status()["active"]
+        elif q < n:
+            time.sleep(min(max(stream[q][0] - now, 0.0), 0.01))
+    end = srv.now() - start
+    lat = []
+    wait = []
+    for r, (arr, _) in zip(ids, stream):
+        ts, ta, td = srv.times(r)
+        lat.append(td - start - arr)
+        wait.append(ta - start - arr)
+    srv.close()
+    lat = np.array(lat)
+    return {
+        "workload": (f"c5: {n} requests, Poisson arrivals at {mtr.synth.SERVING['rate']}/s, xy in "
+                     f"{list(mtr.synth.SERVING['xy'])}, z in {list(mtr.synth.SERVING['z'])}, 1-"
+                     f"{mtr.synth.SERVING['max_axes']} axes over 180 deg, 61 views, M={mtr.synth.SERVING['M']}, "
+                     f"S={mtr.synth.SERVING['S']}"),
+        "requests": n,
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "mean_latency_s": float(lat.mean()),
+        "p50_queue_wait_s": float(np.percentile(wait, 50)),
+        "throughput_req_per_s": n / end,
+        "makespan_s": end,
+        "last_arrival_s": float(stream[-1][0]),
+        "steps": steps,
+        "mean_active_per_step": active_sum / max(steps, 1),
+        "gpu_step_fraction": busy / end,
+        "hbm_budget_gb": args.hbm_budget_gb,
+        "max_active": args.max_active,
+        "generation_s": gen_s,
+    }
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
@@ -87,6 +154,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0 = skip serving)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--calibrate-copy-mb", type=int, default=256,
                     help="size of the streaming copy run once for traffic-counter calibration (0 = skip)")
     args = ap.parse_args()
@@ -203,6 +273,12 @@ def main() -> int:
         log(f"parity sample: {parity}")
     ctx.close()
 
+    serve = None
+    if args.serve_requests > 0 and prec == mtr.PREC_FP32:
+        serve = serving(mtr, args)
+        log(f"serving: p50 {serve['p50_latency_s']:.2f}s p99 {serve['p99_latency_s']:.2f}s, "
+            f"{serve['throughput_req_per_s']:.2f} req/s")
+
     value = steps / wall
     out = {
         "metric": "MACE reconstructions/sec (dual-axis 512x512x128, 2x61 views, 10 MACE x 5 SIRT)",
@@ -228,6 +304,7 @@ def main() -> int:
         "plain_loop": plain,
         "speedup_vs_plain": (value / plain["recon_per_s"]) if plain else None,
         "parity_sample": parity,
+        "serving": serve,
         "copy_calibration": copy_cal,
     }
     print(json.dumps(out), flush=True)

@@ -9,6 +9,8 @@ mirror the workloads the reconstructor is measured on:
       detector, 10 MACE steps of 5 SIRT iterations, lambda 0.5, rho 0.5
   c3  1024x1024x256, 1000 ellipsoids, axes 0/45/90/135 deg, 20 MACE steps
   c4  1024x1024x256, 1000 ellipsoids, axes 0/60/120 deg, 20 MACE steps
+  c5  a stream of requests (serving_stream): Poisson arrivals, uneven
+      volumes and axis counts
 All use Gaussian noise with sigma = 1% of the largest ray sum; agent a of a
 problem seeded s draws its noise from seed 100*s + a.
 """
@@ -163,6 +165,34 @@ CONFIGS = {
                rho=0.5),
     "c4": dict(X=1024, Y=1024, Z=256, n_ell=1000, phis=[0.0, 60.0, 120.0], seed=4, M=20, S=10, lam=0.5, rho=0.5),
 }
+
+
+# Serving stream: 64 requests with Poisson arrivals at mean 4 per second;
+# each draws its volume xy, z and axis count uniformly, axes evenly spaced
+# over 180 degrees, 61 views over +-60 degrees, 10 MACE steps of 5 SIRT.
+SERVING = dict(n=64, rate=4.0, seed=5, xy=(256, 384, 512, 768), z=(64, 128, 256), max_axes=4, M=10, S=5,
+               lam=0.5, rho=0.5, n_ell=24)
+
+
+def serving_stream(n: int = SERVING["n"], rate: float = SERVING["rate"], seed: int = SERVING["seed"],
+                   xy=SERVING["xy"], z=SERVING["z"], max_axes: int = SERVING["max_axes"], M: int = SERVING["M"],
+                   S: int = SERVING["S"], n_ell: int = SERVING["n_ell"], theta=None, threads: int = 0):
+    """[(arrival seconds, Problem)] of a seeded request stream: exponential
+    inter-arrival gaps (-ln(1 - U) / rate) and uniform draws of xy, z and
+    the axis count from the generator's U(0,1) sequence of `seed`."""
+    u = uniform(seed, 4 * n)
+    t = 0.0
+    out = []
+    for q in range(n):
+        t += -np.log1p(-u[4 * q]) / rate
+        s_xy = xy[min(int(u[4 * q + 1] * len(xy)), len(xy) - 1)]
+        s_z = z[min(int(u[4 * q + 2] * len(z)), len(z) - 1)]
+        K = 1 + min(int(u[4 * q + 3] * max_axes), max_axes - 1)
+        phis = [180.0 * a / K for a in range(K)]
+        pb = make_problem(f"req{q}", s_xy, s_xy, s_z, n_ell, phis, seed=seed * 1000 + q, M=M, S=S,
+                          lam=SERVING["lam"], rho=SERVING["rho"], theta=theta, threads=threads)
+        out.append((t, pb))
+    return out
 
 
 def config(name: str, threads: int = 0, **overrides) -> Problem:
