@@ -107,6 +107,11 @@ int mtr_ctx_create(int device, mtr_ctx **out);
 int mtr_ctx_destroy(mtr_ctx *ctx);
 /* Storage precision for agents added afterwards (default FP32). */
 int mtr_ctx_set_precision(mtr_ctx *ctx, int precision);
+/* Fast solves capture one MACE step into a HIP graph and replay it for
+ * every step (default on; off: every launch is issued from the host, as
+ * also happens while per-kernel timing is enabled).  No counterpart in the
+ * reference. */
+int mtr_ctx_use_graphs(mtr_ctx *ctx, int on);
 /* The reconstruction box shared by every agent (the union box the
  * reference computes in synchronizeVol, MultiAxis.cpp:73-99).  Must be set
  * before the first agent is added; allocates w-bar. */

@@ -114,6 +114,9 @@ class Context:
             pass
 
     # -- setup -------------------------------------------------------------
+    def use_graphs(self, on: bool) -> None:
+        _check(lib.mtr_ctx_use_graphs(self._h, int(on)), "use_graphs")
+
     def set_volume(self, vol: VolumeSize) -> None:
         _check(lib.mtr_ctx_set_volume(self._h, _ct.byref(vol)), "set_volume")
         self.vol = vol

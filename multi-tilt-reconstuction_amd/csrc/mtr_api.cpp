@@ -353,6 +353,11 @@ struct mtr_ctx {
     // plain form scratch
     float *d_s = nullptr, *d_cc = nullptr;
     int64_t sc_cap = 0;
+    // graph of one fast MACE step, replayed for every step of a solve
+    bool use_graphs = true;
+    hipGraphExec_t step_graph = nullptr;
+    int graph_S = -1;
+    float graph_lambda = 0.f, graph_rho = 0.f;
     // tuning
     int bp_tz = 8;
     int lds_cap = 6144;
@@ -872,6 +877,10 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         return undo(fail(MTR_ERR_HIP, "volume init failed"));
     int rc = upload_table(ctx);
     if (rc != MTR_OK) return undo(rc);
+    if (ctx->step_graph) {
+        (void)hipGraphExecDestroy(ctx->step_graph);
+        ctx->step_graph = nullptr;
+    }
     if (agent_id) *agent_id = (int)ctx->agents.size() - 1;
     return MTR_OK;
 }
@@ -1066,8 +1075,16 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         (void)hipEventDestroy(pd.b);
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+    return MTR_OK;
+}
+
+int mtr_ctx_use_graphs(mtr_ctx *ctx, int on)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    ctx->use_graphs = on != 0;
     return MTR_OK;
 }
 
@@ -1240,6 +1257,58 @@ int mtr_mace_init(mtr_ctx *ctx)
     return init_from_bpt(ctx);
 }
 
+// One fast MACE step (MultiAxis.cpp:40-71): S SIRT iterations of every
+// agent, then the fused consensus in the canonical layout.  Launches only:
+// no allocation, synchronisation or memset, so it can be captured.
+static int mace_step(mtr_ctx *ctx, int S, float lambda, float rho)
+{
+    const int K = (int)ctx->agents.size();
+    std::vector<int> ids(K);
+    for (int a = 0; a < K; a++) ids[a] = a;
+    std::vector<void *> w, x;
+    for (auto &ag : ctx->agents) {
+        w.push_back(ag.d_w);
+        x.push_back(work_canon(ag));
+    }
+    MTR_TRY(sirt_fast(ctx, ids, S, lambda));
+    for (auto &ag : ctx->agents) MTR_TRY(to_canon(ctx, ag));
+    {
+        Timed t(ctx, MTR_K_CONSENSUS);
+        HIP_TRY(launch_consensus(ctx->precision, K, ctx->N, rho, 1, w.data(), x.data(), ctx->d_wbar,
+                                 ctx->d_cons_partial, consensus_blocks(ctx->N), ctx->stream));
+    }
+    for (auto &ag : ctx->agents) MTR_TRY(from_canon(ctx, ag));
+    return MTR_OK;
+}
+
+// Captures mace_step into a graph (kept until the agents or the solve
+// parameters change) so that a solve replays one graph per step.
+static int step_graph(mtr_ctx *ctx, int S, float lambda, float rho)
+{
+    if (ctx->step_graph && ctx->graph_S == S && ctx->graph_lambda == lambda && ctx->graph_rho == rho)
+        return MTR_OK;
+    if (ctx->step_graph) {
+        (void)hipGraphExecDestroy(ctx->step_graph);
+        ctx->step_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = mace_step(ctx, S, lambda, rho);
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != MTR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIP_TRY(e);
+    const hipError_t ei = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_TRY(ei);
+    ctx->graph_S = S;
+    ctx->graph_lambda = lambda;
+    ctx->graph_rho = rho;
+    return MTR_OK;
+}
+
 int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form, float *wbar_host)
 {
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
@@ -1286,23 +1355,15 @@ int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form
         }
     } else {
         MTR_TRY(mtr_mace_init(ctx));
-        std::vector<int> ids(K);
-        for (int a = 0; a < K; a++) ids[a] = a;
-        std::vector<void *> w, x;
-        for (auto &ag : ctx->agents) {
-            w.push_back(ag.d_w);
-            x.push_back(work_canon(ag));
-        }
+        for (int a = 0; a < K; a++) MTR_TRY(ensure_normaliser(ctx, a));
         const int cb = consensus_blocks(ctx->N);
-        for (int m = 0; m < M; m++) {
-            MTR_TRY(sirt_fast(ctx, ids, S, lambda));
-            for (auto &ag : ctx->agents) MTR_TRY(to_canon(ctx, ag));
-            {
-                Timed t(ctx, MTR_K_CONSENSUS);
-                HIP_TRY(launch_consensus(ctx->precision, K, ctx->N, rho, 1, w.data(), x.data(), ctx->d_wbar,
-                                         ctx->d_cons_partial, cb, ctx->stream));
-            }
-            for (auto &ag : ctx->agents) MTR_TRY(from_canon(ctx, ag));
+        // Per-launch timing records events around every kernel, which a
+        // graph does not hold: timed solves launch eagerly.
+        if (ctx->use_graphs && !ctx->timing && M > 0) {
+            MTR_TRY(step_graph(ctx, S, lambda, rho));
+            for (int m = 0; m < M; m++) HIP_TRY(hipGraphLaunch(ctx->step_graph, ctx->stream));
+        } else {
+            for (int m = 0; m < M; m++) MTR_TRY(mace_step(ctx, S, lambda, rho));
         }
         ctx->cons_blocks = M > 0 ? cb : 0;
     }

@@ -334,3 +334,29 @@ def test_working_volume_layout_round_trip(mtr, phi, kind):
         assert np.array_equal(c.get_volume(mtr.VOL_WORK, 0), v)
         out = c.sirt(0, 0, 0.5, x0=v)
         assert np.array_equal(out, v)
+
+
+def test_graph_replays_match_host_issued_launches(mtr):
+    # A fast solve captures one MACE step into a HIP graph and replays it M
+    # times; every replay count is checked against host-issued launches,
+    # with the context's own eager work (BPT, initial model, host-issued
+    # solves) and synchronisations between the replays.
+    pb = problem(mtr, [0.0, 90.0, 45.0], X=30, Y=26, Z=14, views=9, seed=61)
+    say("graph replays")
+    with load_ctx(mtr, pb) as eager, load_ctx(mtr, pb) as graph:
+        eager.use_graphs(False)
+        for M in (1, 2, 3):
+            ref = eager.mace_solve(M, 3, 0.5, 0.5)
+            got = graph.mace_solve(M, 3, 0.5, 0.5)
+            assert np.array_equal(got, ref), M
+            assert graph.consensus_diff2() == eager.consensus_diff2()
+            for a in range(pb.K):
+                assert np.array_equal(graph.get_volume(mtr.VOL_W, a), eager.get_volume(mtr.VOL_W, a))
+        # host-issued solve on the graph context, then replays again
+        graph.use_graphs(False)
+        mid = graph.mace_solve(2, 3, 0.5, 0.5)
+        graph.use_graphs(True)
+        again = graph.mace_solve(2, 3, 0.5, 0.5)
+        assert np.array_equal(mid, again)
+        # new solve parameters capture a new graph
+        assert np.array_equal(graph.mace_solve(2, 2, 0.4, 0.6), eager.mace_solve(2, 2, 0.4, 0.6))
