@@ -112,6 +112,16 @@ int mtr_ctx_set_precision(mtr_ctx *ctx, int precision);
  * also happens while per-kernel timing is enabled).  No counterpart in the
  * reference. */
 int mtr_ctx_use_graphs(mtr_ctx *ctx, int on);
+/* Back-projection tiles are chosen per agent by timing the candidates on
+ * the agent's own data at its first BPT (mtr_bpt, mtr_mace_init); choices
+ * are cached per shape and geometry, and in the file MTR_TUNE_CACHE names.
+ * Off: fixed defaults.  No counterpart in the reference (fixed 8x8x8
+ * blocks, ReadAndInitialOptions.h:29-31). */
+int mtr_ctx_autotune(mtr_ctx *ctx, int on);
+/* Tile choice of an agent: voxels per thread along the run axis (0 for
+ * the general kernel) and the axis its 64 lanes run along (0 = X,
+ * 1 = Y, 2 = Z). */
+int mtr_agent_tiles(const mtr_ctx *ctx, int agent, int *run, int *lanes_axis);
 /* The reconstruction box shared by every agent (the union box the
  * reference computes in synchronizeVol, MultiAxis.cpp:73-99).  Must be set
  * before the first agent is added; allocates w-bar. */

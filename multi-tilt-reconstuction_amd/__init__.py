@@ -114,6 +114,15 @@ class Context:
             pass
 
     # -- setup -------------------------------------------------------------
+    def autotune(self, on: bool) -> None:
+        _check(lib.mtr_ctx_autotune(self._h, int(on)), "autotune")
+
+    def agent_tiles(self, agent: int) -> tuple[int, int]:
+        """(run length, lane axis) of an agent's back-projection tiles."""
+        r, l = _ct.c_int(), _ct.c_int()
+        _check(lib.mtr_agent_tiles(self._h, agent, _ct.byref(r), _ct.byref(l)), "agent_tiles")
+        return r.value, l.value
+
     def use_graphs(self, on: bool) -> None:
         _check(lib.mtr_ctx_use_graphs(self._h, int(on)), "use_graphs")
 

@@ -8,6 +8,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -213,7 +214,8 @@ struct ColMode {
     int rw = 32;
 };
 
-ColMode col_mode(const double *xc, const double *yc, int A, const mtr_projection_size &prj, int rw)
+ColMode col_mode(const double *xc, const double *yc, int A, const mtr_projection_size &prj, int rw,
+                 int only_p1y = -1)
 {
     ColMode m;
     if (prj.X > fp_col_max_nu()) return m;
@@ -224,6 +226,7 @@ ColMode col_mode(const double *xc, const double *yc, int A, const mtr_projection
     const int waves = bp_col_waves();
     int best = -1;
     for (int p1y = 0; p1y < 2; p1y++) {
+        if (only_p1y >= 0 && p1y != only_p1y) continue;
         const double r1 = std::fabs(yc[1 + p1y]), r2 = std::fabs(yc[2 - p1y]);
         const int rows = (int)std::floor(63 * r1 + (waves - 1) * r2) + 3;
         double wmax = 0.0;
@@ -319,6 +322,7 @@ struct Agent {
     bool p_owned = true;
     float *d_p_std = nullptr;  // [A][nv][nu] copy of p for the plain form (rowfast only)
     ColMode col;               // column plan (oblique in-plane axes)
+    bool tuned = false;        // back-projection tiles chosen by the autotuner
     int2 *d_col_xy = nullptr;
     float2 *d_col_w = nullptr;
     int *d_strip = nullptr;
@@ -353,6 +357,7 @@ struct mtr_ctx {
     // plain form scratch
     float *d_s = nullptr, *d_cc = nullptr;
     int64_t sc_cap = 0;
+    bool autotune = true;
     // graph of one fast MACE step, replayed for every step of a solve
     bool use_graphs = true;
     hipGraphExec_t step_graph = nullptr;
@@ -612,6 +617,169 @@ int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
     return bp_batch(ctx, ctx_batch(ctx, first, count), 0, count, bpt, lambda);
 }
 
+// ---- autotuner ------------------------------------------------------
+// Back-projection tiles (run length along the plane axis or Z, and for the
+// column plan the lane axis) are chosen per agent by timing each candidate
+// on the agent's own data: the initial-model BPT, whose tiles and staging
+// are those of the SIRT back-projection, with HIP events.  Choices are
+// cached per shape and geometry for the process, and in the file named by
+// MTR_TUNE_CACHE when set ("key<TAB>run lanes" lines).
+struct TuneCache {
+    std::mutex mu;
+    bool loaded = false;
+    std::map<std::string, std::pair<int, int>> kv;
+};
+
+TuneCache &tune_cache()
+{
+    static TuneCache c;
+    return c;
+}
+
+uint64_t fnv(const void *p, size_t n, uint64_t h = 1469598103934665603ull)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+std::string tune_key(const mtr_ctx *ctx, const Agent &ag)
+{
+    char buf[256];
+    uint64_t h = fnv(ag.xc.data(), sizeof(double) * ag.xc.size());
+    h = fnv(ag.yc.data(), sizeof(double) * ag.yc.size(), h);
+    std::snprintf(buf, sizeof buf, "%s W%d box%dx%dx%d det%dx%dx%d prec%d geom%016llx", ag.col.on ? "col" : "sep",
+                  ag.bpm.W, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.prj.X, ag.prj.Y, ag.prj.AngN, ctx->precision,
+                  (unsigned long long)h);
+    return buf;
+}
+
+void tune_cache_load(TuneCache &c)
+{
+    if (c.loaded) return;
+    c.loaded = true;
+    const char *path = std::getenv("MTR_TUNE_CACHE");
+    if (!path) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        char *tab = std::strchr(line, '\t');
+        if (!tab) continue;
+        *tab = 0;
+        int run = 0, lanes = 0;
+        if (std::sscanf(tab + 1, "%d %d", &run, &lanes) == 2) c.kv[line] = {run, lanes};
+    }
+    std::fclose(f);
+}
+
+void tune_cache_store(TuneCache &c, const std::string &key, int run, int lanes)
+{
+    c.kv[key] = {run, lanes};
+    if (const char *path = std::getenv("MTR_TUNE_CACHE")) {
+        if (FILE *f = std::fopen(path, "a")) {
+            std::fprintf(f, "%s\t%d %d\n", key.c_str(), run, lanes);
+            std::fclose(f);
+        }
+    }
+}
+
+// Applies a tile choice to agent a (run length, lane axis for the column
+// plan) and refreshes its device tables.
+int apply_tiles(mtr_ctx *ctx, int a, int run, int lanes)
+{
+    Agent &ag = ctx->agents[a];
+    if (ag.col.on) {
+        const ColMode m = col_mode(ag.xc.data(), ag.yc.data(), ag.prj.AngN, ag.prj, run, lanes);
+        if (!m.on) return fail(MTR_ERR_SHAPE, "tile choice does not fit");
+        ag.col = m;
+        const std::vector<BpSepView> bsv = bp_col_views(m, ag.xc.data(), ag.prj.AngN);
+        HIP_TRY(hipMemcpyAsync(ag.d_bsv, bsv.data(), sizeof(BpSepView) * bsv.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    } else {
+        ag.bpm.rw = run;
+    }
+    if (ctx->step_graph) {
+        (void)hipGraphExecDestroy(ctx->step_graph);
+        ctx->step_graph = nullptr;
+    }
+    return upload_table(ctx);
+}
+
+// Chooses the back-projection tiles of agent a by timing SIRT
+// back-projections of a zeroed residual (x += 0).  Clobbers the agent's
+// residual: call only before a BPT, which needs none.
+int tune_agent(mtr_ctx *ctx, int a)
+{
+    Agent &ag = ctx->agents[a];
+    if (ag.tuned || !ctx->autotune || !(ag.col.on || ag.bpm.sep)) {
+        ag.tuned = true;
+        return MTR_OK;
+    }
+    std::vector<std::pair<int, int>> cand;
+    if (ag.col.on) {
+        for (int run : {32, 16})
+            for (int lanes : {0, 1})
+                if (col_mode(ag.xc.data(), ag.yc.data(), ag.prj.AngN, ag.prj, run, lanes).on) cand.push_back({run, lanes});
+    } else {
+        for (int run : {32, 16, 8}) cand.push_back({run, 0});
+    }
+    TuneCache &tc = tune_cache();
+    const std::string key = tune_key(ctx, ag);
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        tune_cache_load(tc);
+        auto it = tc.kv.find(key);
+        if (it != tc.kv.end()) {
+            ag.tuned = true;
+            return apply_tiles(ctx, a, it->second.first, it->second.second);
+        }
+    }
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    HIP_TRY(launch_fill(ctx->precision, ag.P(), ag.d_r, 0.0f, ctx->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    std::pair<int, int> pick = cand.front();
+    int rc = MTR_OK;
+    for (const auto &c : cand) {
+        if ((rc = apply_tiles(ctx, a, c.first, c.second)) != MTR_OK) break;
+        if ((rc = bp_group(ctx, a, 1, false, 0.0f)) != MTR_OK) break;  // warm-up
+        float t = 1e30f;
+        for (int rep = 0; rep < 3 && rc == MTR_OK; rep++) {
+            float tr = 0.f;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess || bp_group(ctx, a, 1, false, 0.0f) != MTR_OK ||
+                hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&tr, e0, e1) != hipSuccess)
+                rc = fail(MTR_ERR_HIP, "autotuner timing failed");
+            t = std::min(t, tr);
+        }
+        if (rc != MTR_OK) break;
+        if (t < best) {
+            best = t;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timing = timing;
+    if (rc != MTR_OK) return rc;
+    ag.tuned = true;
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        tune_cache_store(tc, key, pick.first, pick.second);
+    }
+    return apply_tiles(ctx, a, pick.first, pick.second);
+}
+
+int tune_all(mtr_ctx *ctx)
+{
+    for (int a = 0; a < (int)ctx->agents.size(); a++) MTR_TRY(tune_agent(ctx, a));
+    return MTR_OK;
+}
+
 int ensure_plain_scratch(mtr_ctx *ctx, int64_t P)
 {
     if (ctx->sc_cap >= P) return MTR_OK;
@@ -734,14 +902,12 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     for (int v = 0; v < A; v++) all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     ag.lanes_k = all_wx;
     ag.bpm = bp_mode(plan, xc, yc, ctx->vol, *prj);
-    if (const char *e = std::getenv("MTR_BP_RW")) ag.bpm.rw = std::atoi(e) == 8 ? 8 : (std::atoi(e) == 16 ? 16 : 32);
     int sW = 0;
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     ag.fast = ag.fp_nc ? sW : 0;
     if (!ag.fp_nc && !ag.bpm.sep) {
-        int rw = 32;
-        if (const char *e = std::getenv("MTR_COL_RW")) rw = std::atoi(e) == 16 ? 16 : 32;
-        ag.col = col_mode(xc, yc, A, *prj, rw);
+        ag.col = col_mode(xc, yc, A, *prj, 32);
+        if (!ag.col.on) ag.col = col_mode(xc, yc, A, *prj, 16);
         if (ag.col.on) ag.fast = 2;  // Z-contiguous columns
     }
     layout_strides(ag.fast, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st);
@@ -1081,6 +1247,22 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     return MTR_OK;
 }
 
+int mtr_ctx_autotune(mtr_ctx *ctx, int on)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    ctx->autotune = on != 0;
+    return MTR_OK;
+}
+
+int mtr_agent_tiles(const mtr_ctx *ctx, int agent, int *run, int *lanes_axis)
+{
+    MTR_TRY(check_agent(const_cast<mtr_ctx *>(ctx), agent));
+    const Agent &ag = ctx->agents[agent];
+    if (run) *run = ag.col.on ? ag.col.rw : (ag.bpm.sep ? ag.bpm.rw : 0);
+    if (lanes_axis) *lanes_axis = ag.col.on ? ag.col.p1y : (ag.bpm.sep ? ag.bpm.P1 : 0);
+    return MTR_OK;
+}
+
 int mtr_ctx_use_graphs(mtr_ctx *ctx, int on)
 {
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
@@ -1162,6 +1344,7 @@ int mtr_bpt(mtr_ctx *ctx, int agent, float *out_host)
 {
     MTR_TRY(check_agent(ctx, agent));
     MTR_TRY(set_device(ctx));
+    MTR_TRY(tune_agent(ctx, agent));
     MTR_TRY(bp_group(ctx, agent, 1, true, 1.0f));
     if (out_host) {
         MTR_TRY(to_canon(ctx, ctx->agents[agent]));
@@ -1252,6 +1435,7 @@ int mtr_mace_init(mtr_ctx *ctx)
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
     if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
     MTR_TRY(set_device(ctx));
+    MTR_TRY(tune_all(ctx));
     // BPT of every agent into its working volume, batched per kernel kind.
     MTR_TRY(bp_group(ctx, 0, (int)ctx->agents.size(), true, 1.0f));
     return init_from_bpt(ctx);

@@ -360,3 +360,18 @@ def test_graph_replays_match_host_issued_launches(mtr):
         assert np.array_equal(mid, again)
         # new solve parameters capture a new graph
         assert np.array_equal(graph.mace_solve(2, 2, 0.4, 0.6), eager.mace_solve(2, 2, 0.4, 0.6))
+
+
+def test_autotuned_tiles_do_not_change_results(mtr):
+    # the tiles only regroup work: every voxel still sums its views in order
+    pb = problem(mtr, [0.0, 90.0, 60.0, 135.0], X=70, Y=66, Z=40, views=9, seed=62)
+    say("autotune")
+    with load_ctx(mtr, pb) as tuned, load_ctx(mtr, pb) as fixed:
+        fixed.autotune(False)
+        a = tuned.mace_solve(2, 2, 0.5, 0.5)
+        b = fixed.mace_solve(2, 2, 0.5, 0.5)
+        say("tiles", [tuned.agent_tiles(q) for q in range(pb.K)], [fixed.agent_tiles(q) for q in range(pb.K)])
+        assert np.array_equal(a, b)
+        for q in range(pb.K):
+            run, lanes = tuned.agent_tiles(q)
+            assert run in (0, 8, 16, 32) and lanes in (0, 1, 2)  # 0: the general kernel
