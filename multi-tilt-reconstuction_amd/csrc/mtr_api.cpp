@@ -327,6 +327,7 @@ struct Agent {
     float2 *d_col_w = nullptr;
     int *d_strip = nullptr;
     float *d_pm = nullptr, *d_pn = nullptr;
+    float *d_bpc = nullptr;    // column plan: BP normaliser per voxel
     float *d_c = nullptr;
     void *d_r = nullptr;
     void *d_x = nullptr;
@@ -472,6 +473,7 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.col_pitch = ag.col.pitch;
     d.pm = ag.d_pm;
     d.pn = ag.d_pn;
+    d.bpc = ag.d_bpc;
     return d;
 }
 
@@ -506,6 +508,9 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
             HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), ag.prj.X, true, ctx->stream));
             HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + a, 1, fp_col_res_blocks(d), 2, ag.d_c,
                                       ctx->stream));
+            // and the back-projection normaliser of every voxel
+            HIP_TRY(launch_bp_col(ctx->precision, ctx->d_ags + a, 1, bp_col_blocks(d, ag.col.rw), 2, 0.0f, ag.col.rw,
+                                  ctx->stream));
         } else {
             HIP_TRY(launch_fp(ctx->precision, ctx->d_ags + a, 1, ag.nfp, 2, ag.lanes_k, nullptr, ag.d_c,
                               ctx->stream));
@@ -590,7 +595,7 @@ int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float
         }
         Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP);
         if (cm.on) {
-            HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, cm.rw, eng->stream));
+            HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, eng->stream));
         } else if (m.sep) {
             HIP_TRY(launch_bp_sep(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, m.W, m.P1, m.P2, m.rw,
                                   eng->stream));
@@ -718,7 +723,7 @@ int tune_agent(mtr_ctx *ctx, int a)
     }
     std::vector<std::pair<int, int>> cand;
     if (ag.col.on) {
-        for (int run : {32, 16})
+        for (int run : {16})
             for (int lanes : {0, 1})
                 if (col_mode(ag.xc.data(), ag.yc.data(), ag.prj.AngN, ag.prj, run, lanes).on) cand.push_back({run, lanes});
     } else {
@@ -888,7 +893,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         Agent &g = ctx->agents.back();
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
-                        (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn})
+                        (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
+                        (void *)g.d_bpc})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -906,8 +912,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     ag.fast = ag.fp_nc ? sW : 0;
     if (!ag.fp_nc && !ag.bpm.sep) {
-        ag.col = col_mode(xc, yc, A, *prj, 32);
-        if (!ag.col.on) ag.col = col_mode(xc, yc, A, *prj, 16);
+        ag.col = col_mode(xc, yc, A, *prj, 16);
         if (ag.col.on) ag.fast = 2;  // Z-contiguous columns
     }
     layout_strides(ag.fast, ctx->vol.X, ctx->vol.Y, ctx->vol.Z, ag.st);
@@ -969,6 +974,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         ALLOC(ag.d_strip, sizeof(int) * cnt.size());
         ALLOC(ag.d_pm, sizeof(float) * P);
         ALLOC(ag.d_pn, sizeof(float) * P);
+        ALLOC(ag.d_bpc, sizeof(float) * ctx->N);
         if (hipMemcpy(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ag.d_col_w, wts.data(), sizeof(float2) * wts.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ag.d_strip, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -1230,7 +1236,8 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     for (auto &ag : ctx->agents) {
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
-                        (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn})
+                        (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
+                        (void *)ag.d_bpc})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
@@ -1324,10 +1331,10 @@ int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size
     const size_t cols = (size_t)ctx->vol.X * ctx->vol.Y;
     const size_t nfp = (size_t)((prj->X + 63) / 64 + 16) * ((prj->Y + 3) / 4 + 16) * prj->AngN;
     // geometry tables, p (plus a converted copy), c, r, the partial rows of
-    // the column plan, x, w and a canonical copy of x, column tables, FP64
-    // partial sums
+    // the column plan, x, w and a canonical copy of x, the column plan's
+    // back-projection normaliser and tables, FP64 partial sums
     *bytes = (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj->AngN + 2 * 4 * P + 4 * P + es * P +
-             2 * 4 * P + 3 * es * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp;
+             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp;
     return MTR_OK;
 }
 
@@ -1344,6 +1351,7 @@ int mtr_bpt(mtr_ctx *ctx, int agent, float *out_host)
 {
     MTR_TRY(check_agent(ctx, agent));
     MTR_TRY(set_device(ctx));
+    MTR_TRY(ensure_normaliser(ctx, agent));
     MTR_TRY(tune_agent(ctx, agent));
     MTR_TRY(bp_group(ctx, agent, 1, true, 1.0f));
     if (out_host) {
@@ -1435,6 +1443,7 @@ int mtr_mace_init(mtr_ctx *ctx)
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
     if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
     MTR_TRY(set_device(ctx));
+    for (int a = 0; a < (int)ctx->agents.size(); a++) MTR_TRY(ensure_normaliser(ctx, a));
     MTR_TRY(tune_all(ctx));
     // BPT of every agent into its working volume, batched per kernel kind.
     MTR_TRY(bp_group(ctx, 0, (int)ctx->agents.size(), true, 1.0f));
@@ -1871,7 +1880,7 @@ size_t request_bytes(int precision, const mtr_volume_size &vol, int K, const mtr
         const size_t P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
         const size_t nfp = (size_t)((prj[a].X + 63) / 64 + 16) * ((prj[a].Y + 3) / 4 + 16) * prj[a].AngN;
         b += (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj[a].AngN + 2 * 4 * P + 4 * P + es * P +
-             2 * 4 * P + 3 * es * N + 12 * cols + 4 * (size_t)(prj[a].Y + 2) + 8 * nfp;
+             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj[a].Y + 2) + 8 * nfp;
     }
     return b;
 }

@@ -80,6 +80,7 @@ struct AgentDev {
     int col_p1y;           // back-projection tiles: lanes along Y (1) or X (0)
     int col_pitch;         // back-projection LDS pitch (pairs per staged row)
     float *pm, *pn;        // FP partial rows [A][nv][nu]: from strip k and strip k - 1
+    float *bpc;            // back-projection normaliser C of every voxel (working layout)
 };
 
 }  // namespace mtr

@@ -30,12 +30,14 @@ hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_block
 int fp_col_res_blocks(const AgentDev &ag);
 hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
                              hipStream_t st);
-// Back-projection with runs of rw (16 or 32) voxels along Z; the staged
-// footprint (rows x col_pitch pairs) must fit col_elems().
+// Back-projection with runs of rw = 16 voxels along Z; the staged
+// footprint (rows x col_pitch pairs) must fit col_elems().  mode 0: SIRT
+// update, 1: BPT, 2: the geometry-only normaliser C into ag.bpc (needed by
+// modes 0 and 1).
 int bp_col_blocks(const AgentDev &ag, int rw);
 int bp_col_waves();
 int col_elems();
-hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda, int rw,
                          hipStream_t st);
 
 // Forward projection of nag agents (device table d_ags), mode 0 = fused

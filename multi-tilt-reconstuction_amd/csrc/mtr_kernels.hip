@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace mtr {
 
 // ------------------------------------------------------------------------
@@ -1061,31 +1063,42 @@ __global__ __launch_bounds__(256) void k_fp_col_res(const AgentDev *__restrict__
 // launch; per view the block stages them over the tile's u range as
 // (r[i], r[i+1] - r[i]) pairs (zero outside the detector, double
 // buffered), and voxel t of the run gathers both rows at u0 + t xc3.
-// The normaliser C of a voxel is (wk0 + wk1) times the summed u weights:
-// one per view less the weight of taps that fall off the detector's sides.
+// The normaliser C of a voxel is geometry only: (wk0 + wk1) times the
+// summed u weights, one per view less the weight of taps that fall off the
+// detector's sides.  MODE 2 computes it once into ag.bpc; MODE 0 (SIRT
+// update) and MODE 1 (BPT) read it.
 constexpr int kColBpWaves = 8;
 constexpr int kColElems = 7 * 64 * kColBpWaves;  // staged pairs per view (rows x pitch)
 
 int col_elems() { return kColElems; }
 
-template <class T, int RW, bool BPT>
+template <class T, int RW, int MODE>
 __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__restrict__ ags, float lambda)
 {
+    constexpr bool BPT = MODE == 1, NORM = MODE == 2;
     constexpr int NT = 64 * kColBpWaves, kPer = kColElems / NT;
-    constexpr int kStage = 2 * kColElems, kEpi = kColBpWaves * (64 * 17 + 64) / 2;
+    constexpr int kStage = 2 * kColElems, kEpi = kColBpWaves * 64 * 17 / 2;
     __shared__ float2 lds_raw[kStage > kEpi ? kStage : kEpi];
     const AgentDev &ag = ags[blockIdx.y];
     const bool p1y = ag.col_p1y != 0;
     const int pitch = ag.col_pitch;
     const int n1 = p1y ? ag.Y : ag.X, n2 = p1y ? ag.X : ag.Y;
     const int s1 = p1y ? ag.Ys : ag.Xs, s2 = p1y ? ag.Xs : ag.Ys;
-    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kColBpWaves - 1) / kColBpWaves, ntz = (ag.Z + RW - 1) / RW;
+    // runs start at absolute multiples of RW, so that a voxel's run (and
+    // its arithmetic) does not depend on the box, e.g. on a Z-slab split
+    const int zal = ag.Zs >= 0 ? (ag.Zs / RW) * RW : -(((-ag.Zs) + RW - 1) / RW) * RW;
+    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kColBpWaves - 1) / kColBpWaves, ntz = (ag.Ze - zal + RW - 1) / RW;
     const int nb = nt1 * nt2 * ntz, per = (nb + 7) / 8;
     const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
     const int bz = b % ntz, b1 = (b / ntz) % nt1, b2 = b / (ntz * nt1);
     const int lane = threadIdx.x & 63, wid = wave_id();
-    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kColBpWaves, Z0 = ag.Zs + bz * RW;
+    const int P10 = s1 + b1 * 64, P20 = s2 + b2 * kColBpWaves, Z0 = zal + bz * RW;
+    // u is stepped in FP32 from the last multiple of 32 planes at or below
+    // Z0, where it is formed by the reference's FP64 chain: the same
+    // arithmetic for runs of 16 and 32
+    const int Z32 = Z0 >= 0 ? (Z0 / 32) * 32 : -(((-Z0) + 31) / 32) * 32;
+    const int zlo = max(ag.Zs - Z0, 0);
     const int p1 = P10 + lane, p2 = P20 + wid;
     const int e1 = min(s1 + n1 - 1, P10 + 63), e2 = min(s2 + n2 - 1, P20 + kColBpWaves - 1);
     const int ew = min(RW, ag.Ze - Z0) - 1;
@@ -1113,9 +1126,13 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     const auto src = gview(reinterpret_cast<const T *>(BPT ? ag.p : ag.r));
     const int64_t plane = (int64_t)nu * nv;
 
-    float S[RW], D[RW];  // D: summed u weight of taps off the detector
+    // S: gathered sums; D (MODE 2): summed u weight of taps off the detector
+    float S[NORM ? 1 : RW], D[NORM ? RW : 1];
 #pragma unroll
-    for (int t = 0; t < RW; t++) S[t] = D[t] = 0.0f;
+    for (int t = 0; t < RW; t++) {
+        if constexpr (!NORM) S[t] = 0.0f;
+        if constexpr (NORM) D[t] = 0.0f;
+    }
     float2 pre[kPer];
 
     // detector columns of the tile in view a: u at the tile origin plus
@@ -1152,67 +1169,74 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
 
     int fi0, fi1;
     window(0, fi0, fi1);
-    fetch(0, fi0, fi1);
-    commit(0);
-    __syncthreads();
+    if constexpr (!NORM) {
+        fetch(0, fi0, fi1);
+        commit(0);
+        __syncthreads();
+    }
     for (int a = 0; a < ag.A; a++) {
         const int cfi0 = fi0, cfi1 = fi1;
         if (a + 1 < ag.A) {
             window(a + 1, fi0, fi1);
-            fetch(a + 1, fi0, fi1);
+            if constexpr (!NORM) fetch(a + 1, fi0, fi1);
         }
-        // u of every voxel by the reference's FP64 chain (so a voxel's
-        // value does not depend on where its run starts)
         const ViewCoef vc = ag.coef[a];
-        const double bxy = fma(vc.c[2], (double)Y, fma(vc.c[1], (double)X, vc.c[0]));
-        const double c3 = vc.c[3];
+        const double ub = fma(vc.c[3], (double)Z32, fma(vc.c[2], (double)Y, fma(vc.c[1], (double)X, vc.c[0]))) - 1.0;
+        const double fb = floor(ub);
+        const float f0 = (float)(ub - fb), step = (float)vc.c[3];
+        const int ib = (int)fb - cfi0;
         const float2 *row0 = lds_raw + (a & 1) * kColElems + lr;
         const float2 *row1 = row0 + pitch;
         const int lmax = pitch - 1;
-        if (cfi0 >= 0 && cfi1 + 1 <= nu - 1) {
+        // tap t of the run is plane Z0 + t = Z32 + OFF + t (OFF = 0, or 16
+        // for the upper half of a 32-plane group with runs of 16)
+        auto taps = [&](auto off_c) {
+            constexpr int OFF = decltype(off_c)::value;
+            if constexpr (!NORM) {
+                // zero pairs outside the detector make the same update
+                // exact at its side edges
 #pragma unroll
-            for (int t = 0; t < RW; t++) {
-                const double u = fma(c3, (double)(Z0 + t), bxy) - 1.0;
-                const double fl = floor(u);
-                const float du = (float)(u - fl);
-                const int li = min(max((int)fl - cfi0, 0), lmax);
-                const float2 qa = row0[li], qb = row1[li];
-                S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
-                S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
-            }
-        } else {
+                for (int t = 0; t < RW; t++) {
+                    const float g = fmaf((float)(OFF + t), step, f0);
+                    const float fg = floorf(g);
+                    const float du = g - fg;
+                    const int li = min(max(ib + (int)fg, 0), lmax);
+                    const float2 qa = row0[li], qb = row1[li];
+                    S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
+                    S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
+                    // at most 8 taps' LDS reads in flight (registers)
+                    if ((t & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+                }
+            } else if (!(cfi0 >= 0 && cfi1 + 1 <= nu - 1)) {
 #pragma unroll
-            for (int t = 0; t < RW; t++) {
-                const double u = fma(c3, (double)(Z0 + t), bxy) - 1.0;
-                const double fl = floor(u);
-                const float du = (float)(u - fl);
-                const int i0 = (int)fl;
-                const int li = min(max(i0 - cfi0, 0), lmax);
-                const float2 qa = row0[li], qb = row1[li];
-                S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
-                S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
-                // weight of the taps that fall off the detector's sides
-                const bool v0 = (unsigned)i0 < (unsigned)nu, v1 = (unsigned)(i0 + 1) < (unsigned)nu;
-                D[t] += v0 ? (v1 ? 0.0f : du) : (v1 ? 1.0f - du : 1.0f);
+                for (int t = 0; t < RW; t++) {
+                    const float g = fmaf((float)(OFF + t), step, f0);
+                    const float fg = floorf(g);
+                    const float du = g - fg;
+                    const int i0 = ib + cfi0 + (int)fg;
+                    // weight of the taps that fall off the detector's sides
+                    const bool v0 = (unsigned)i0 < (unsigned)nu, v1 = (unsigned)(i0 + 1) < (unsigned)nu;
+                    D[t] += v0 ? (v1 ? 0.0f : du) : (v1 ? 1.0f - du : 1.0f);
+                }
             }
+        };
+        if (RW == 16 && Z0 != Z32) taps(std::integral_constant<int, 16>{});
+        else taps(std::integral_constant<int, 0>{});
+        if constexpr (!NORM) {
+            if (a + 1 < ag.A) commit((a + 1) & 1);
+            __syncthreads();
         }
-        if (a + 1 < ag.A) commit((a + 1) & 1);
-        __syncthreads();
     }
-    // Epilogue: q = S/C per voxel and a bit per voxel for C != 0, then the
-    // runs are stored transposed through LDS (16 lanes per 16 consecutive
-    // voxels of one run) as in the separable back-projection.
-    float *ep = reinterpret_cast<float *>(lds_raw) + wid * (64 * 17 + 64);
-    int *mk = reinterpret_cast<int *>(ep + 64 * 17);
-    const float kw = wk0 + wk1;
-    uint32_t mask = 0;
+    if constexpr (NORM) __syncthreads();  // the epilogue reuses the staging LDS
+    // Epilogue: the runs are stored transposed through LDS (16 lanes per 16
+    // consecutive voxels of one run) as in the separable back-projection;
+    // MODE 2 stores C, the others S/C of each voxel with C read back.
+    float *ep = reinterpret_cast<float *>(lds_raw) + wid * (64 * 17);
+    if constexpr (NORM) {
+        const float kw = wk0 + wk1;
 #pragma unroll
-    for (int t = 0; t < RW; t++) {
-        const float C = kw * ((float)ag.A - D[t]);
-        S[t] = C != 0.0f ? S[t] / C : 0.0f;
-        mask |= (C != 0.0f ? 1u : 0u) << t;
+        for (int t = 0; t < RW; t++) D[t] = kw * ((float)ag.A - D[t]);
     }
-    mk[lane] = (int)mask;
     T *x = reinterpret_cast<T *>(ag.x);
     const int64_t sP1 = p1y ? ag.sy : ag.sx;
     const int X0 = p1y ? p2 : P10, Y0 = p1y ? P10 : p2;
@@ -1223,20 +1247,25 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
 #pragma unroll
     for (int h = 0; h < RW / CH; h++) {
 #pragma unroll
-        for (int t = 0; t < CH; t++) ep[lane * 17 + t] = S[CH * h + t];
+        for (int t = 0; t < CH; t++) {
+            if constexpr (NORM) ep[lane * 17 + t] = D[CH * h + t];
+            else ep[lane * 17 + t] = S[CH * h + t];
+        }
         wave_lds_sync();
         const int tt = lane % CH, tw = CH * h + tt;
 #pragma unroll 4
         for (int i = 0; i < CH; i++) {
             const int line = i * LPI + lane / CH;
             const float qv = ep[line * 17 + tt];
-            const bool cnz = (mk[line] >> tw) & 1;
-            if (wave_ok && tw <= ew && P10 + line <= e1) {
+            if (wave_ok && tw >= zlo && tw <= ew && P10 + line <= e1) {
                 const int64_t j = jb + line * sP1 + tw * ag.sz;
-                if (BPT) {
-                    stf<T>(x, j, qv);
-                } else if (cnz) {
-                    stf<T>(x, j, fmaf(qv, lambda, ldf<T>(x, j)));
+                if constexpr (NORM) {
+                    ag.bpc[j] = qv;
+                } else {
+                    const float C = ag.bpc[j];
+                    const float q = C != 0.0f ? qv / C : 0.0f;
+                    if (BPT) stf<T>(x, j, q);
+                    else if (C != 0.0f) stf<T>(x, j, fmaf(q, lambda, ldf<T>(x, j)));
                 }
             }
         }
@@ -1958,28 +1987,31 @@ int bp_col_blocks(const AgentDev &ag, int rw)
 int bp_col_waves() { return kColBpWaves; }
 
 template <class T, int RW>
-static void launch_bp_col_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, hipStream_t st)
+static void launch_bp_col_rw(dim3 grid, const AgentDev *d_ags, int mode, float lambda, hipStream_t st)
 {
-    if (bpt) hipLaunchKernelGGL((k_bp_col<T, RW, true>), grid, dim3(64 * kColBpWaves), 0, st, d_ags, lambda);
-    else hipLaunchKernelGGL((k_bp_col<T, RW, false>), grid, dim3(64 * kColBpWaves), 0, st, d_ags, lambda);
+    const dim3 block(64 * kColBpWaves);
+    if (mode == 2) hipLaunchKernelGGL((k_bp_col<T, RW, 2>), grid, block, 0, st, d_ags, lambda);
+    else if (mode == 1) hipLaunchKernelGGL((k_bp_col<T, RW, 1>), grid, block, 0, st, d_ags, lambda);
+    else hipLaunchKernelGGL((k_bp_col<T, RW, 0>), grid, block, 0, st, d_ags, lambda);
 }
 
 template <class T>
-static hipError_t launch_bp_col_t(const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+static hipError_t launch_bp_col_t(const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda, int rw,
                                   hipStream_t st)
 {
+    // runs of 16 planes (runs of 32 need more than 128 VGPRs per lane)
+    if (rw != 16) return hipErrorInvalidValue;
     dim3 grid(max_blocks, nag);
-    if (rw == 32) launch_bp_col_rw<T, 32>(grid, d_ags, bpt, lambda, st);
-    else launch_bp_col_rw<T, 16>(grid, d_ags, bpt, lambda, st);
+    launch_bp_col_rw<T, 16>(grid, d_ags, mode, lambda, st);
     return hipGetLastError();
 }
 
-hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int rw,
+hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda, int rw,
                          hipStream_t st)
 {
-    if (prec == MTR_PREC_BF16) return launch_bp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
-    if (prec == MTR_PREC_FP16) return launch_bp_col_t<__half>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
-    return launch_bp_col_t<float>(d_ags, nag, max_blocks, bpt, lambda, rw, st);
+    if (prec == MTR_PREC_BF16) return launch_bp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, mode, lambda, rw, st);
+    if (prec == MTR_PREC_FP16) return launch_bp_col_t<__half>(d_ags, nag, max_blocks, mode, lambda, rw, st);
+    return launch_bp_col_t<float>(d_ags, nag, max_blocks, mode, lambda, rw, st);
 }
 
 hipError_t launch_reduce(const double *part, int64_t n, double *out, hipStream_t st)
