@@ -723,14 +723,17 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     // and consecutive rows in the row-fastest [nu][nv] layout, so the
     // staging loads are contiguous either way.
     const bool rowfast = ag.pk == 1;
-    int srow[St::kPer], scol[St::kPer];
+    int srow[St::kPer], scol[St::kPer], coff[St::kPer];
 #pragma unroll
     for (int q = 0; q < St::kPer; q++) {
         const int e = threadIdx.x + NT * q;
         const int r = rowfast ? e % RW : e / PITCH, cidx = rowfast ? e / RW : e % PITCH;
         srow[q] = e < St::kElems ? r : RW;  // RW: no slot
         scol[q] = cidx;
+        coff[q] = cidx * ag.pi;
     }
+    // full runs: every staged row is on the detector (host-checked)
+    const bool full_run = ew == RW - 1;
 
     auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
         const BpSepView bv = ag.bsv[a];
@@ -741,6 +744,18 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         k0row = bv.k0 + bv.kstep * W0;
         kstep = bv.kstep;
         const uint32_t vbase = (uint32_t)(plane * a);
+        if (full_run && fi0 >= 0 && fi0 + PITCH <= nu - 1) {
+            // every slot's pair lies inside the detector: no clamps or masks
+            const int base = (int)vbase + k0row * ag.pk + fi0 * ag.pi, rs = kstep * ag.pk;
+#pragma unroll
+            for (int q = 0; q < St::kPer; q++) {
+                const int o = base + (srow[q] < RW ? srow[q] : 0) * rs + coff[q];
+                const float v0 = ldg<T>(src, (uint32_t)o);
+                const float v1 = ldg<T>(src, (uint32_t)(o + ag.pi));
+                pre[q] = make_float2(v0, v1 - v0);
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < St::kPer; q++) {
             // branch-free: clamp the address into the detector, select 0 outside
@@ -1145,9 +1160,25 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
         fi0 = (int)floor(ut + bv.umin) - 1;
         fi1 = (int)floor(ut + bv.umax) + 1;
     };
+    // rows the staging slots cover; a footprint whose slots all lie inside
+    // the detector is staged without clamps or masks
+    const int slot_rows = kColElems / pitch;
+    const bool rows_inside = klo >= 0 && klo + slot_rows <= nv - 1;
     auto fetch = [&](int a, int fi0, int fi1) {
         const int width = fi1 - fi0 + 1;
         const uint32_t vbase = (uint32_t)(plane * a);
+        if (rows_inside && fi0 >= 0 && fi0 + pitch <= nu - 1) {
+            const uint32_t base = vbase + (uint32_t)klo * (uint32_t)nu + (uint32_t)fi0;
+#pragma unroll
+            for (int q = 0; q < kPer; q++) {
+                const int e = threadIdx.x + NT * q;
+                const int rr = e / pitch, cc = e - rr * pitch;
+                const uint32_t o = base + (uint32_t)(rr * nu + cc);
+                const float v0 = ldg<T>(src, o), v1 = ldg<T>(src, o + 1);
+                pre[q] = make_float2(v0, v1 - v0);
+            }
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
             const int e = threadIdx.x + NT * q;
