@@ -156,7 +156,8 @@ def main() -> int:
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0 = skip serving)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
-    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--max-active", type=int, default=8,
+                    help="concurrently active requests (8: lowest p99 on the seeded stream, 2-64 swept)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=256,
                     help="size of the streaming copy run once for traffic-counter calibration (0 = skip)")
     args = ap.parse_args()
