@@ -145,3 +145,34 @@ def test_compat_symbols_are_exported(mtr):
                  "reconstructModel", "divideSlice", "synchronizeVol", "readOptions", "readMrcHeader",
                  "readMrcAndTXBR", "updataAndWriteModel"):
         assert f"mtr_compat::{name}(" in out, name
+
+
+def test_serving_stream_is_seeded_and_shaped(mtr):
+    a = mtr.synth.serving_stream(n=12, xy=(16, 24), z=(8, 12), theta=[-30.0, 0.0, 30.0])
+    b = mtr.synth.serving_stream(n=12, xy=(16, 24), z=(8, 12), theta=[-30.0, 0.0, 30.0])
+    assert [t for t, _ in a] == [t for t, _ in b]
+    assert all(np.array_equal(p.ps[0], q.ps[0]) for (_, p), (_, q) in zip(a, b))
+    times = [t for t, _ in a]
+    assert all(t1 > t0 for t0, t1 in zip(times, times[1:]))  # arrivals increase
+    for _, pb in a:
+        assert pb.vol.X == pb.vol.Y and pb.vol.X in (16, 24) and pb.vol.Z in (8, 12)
+        assert 1 <= pb.K <= 4 and pb.M == 10 and pb.S == 5
+        # axes evenly spaced over 180 degrees: the v gradient turns by 180/K
+        ang = [np.degrees(np.arctan2(-yc[1], yc[2])) % 180.0 for yc in pb.ycs]
+        assert np.allclose(np.diff(ang), 180.0 / pb.K) if pb.K > 1 else True
+
+
+def test_server_without_device_is_an_error(mtr):
+    if mtr.gpu_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(mtr.MtrError) as ei:
+        mtr.Server(0)
+    assert ei.value.code == -8
+
+
+def test_server_argument_checks(mtr):
+    import ctypes as ct
+    out = ct.c_void_p()
+    assert mtr.lib.mtr_server_create(0, 7, 1 << 30, 4, ct.byref(out)) == -1  # unknown precision
+    assert mtr.lib.mtr_server_create(0, 0, 0, 4, ct.byref(out)) == -1        # empty budget
+    assert mtr.lib.mtr_server_create(0, 0, 1 << 30, 0, ct.byref(out)) == -1  # no slots
