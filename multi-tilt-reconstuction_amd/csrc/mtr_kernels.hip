@@ -930,27 +930,131 @@ __device__ __forceinline__ void seg_linear(float &a, float &b, int key)
     b = sb;
 }
 
-// Segmented inclusive scan over the lanes of a wave for segments of at
-// most maxrun lanes: up to 4 lanes by seg_linear; longer segments by
-// Hillis-Steele inside each row of 16 lanes (row_shr 1, 2, 4, 8) and the
-// row carries (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2
-// and 3).  The last lane of a segment ends with the segment's sums.
-__device__ __forceinline__ void seg_scan(float &a, float &b, int key, int maxrun)
+// Segmented inclusive scan over the lanes of a wave, for segments of at
+// most 1 + CLS lanes (CLS = 1, 2, 3: seg_linear), at most 8 (CLS = 8) or
+// any length (CLS = 64): Hillis-Steele inside each row of 16 lanes
+// (row_shr 1, 2, 4, and 8 when CLS = 64) and the row carries (row_bcast 15
+// into rows 1 and 3, row_bcast 31 into rows 2 and 3).  The last lane of a
+// segment ends with the segment's sums.
+template <int CLS>
+__device__ __forceinline__ void seg_scan_cls(float &a, float &b, int key)
 {
-    if (maxrun <= 2) {
-        seg_linear<1>(a, b, key);
-    } else if (maxrun == 3) {
-        seg_linear<2>(a, b, key);
-    } else if (maxrun == 4) {
-        seg_linear<3>(a, b, key);
+    if constexpr (CLS <= 3) {
+        seg_linear<CLS>(a, b, key);
     } else {
         seg_step<0x111>(a, b, key);
         seg_step<0x112>(a, b, key);
         seg_step<0x114>(a, b, key);
-        if (maxrun > 8) seg_step<0x118>(a, b, key);
+        if constexpr (CLS > 8) seg_step<0x118>(a, b, key);
         seg_carry<0x142, 0xa>(a, b, key);
         seg_carry<0x143, 0xc>(a, b, key);
     }
+}
+
+// One chunk of 64 planes of one column: lane zi - zc holds plane zi with
+// value xv.  Leaders (last lane of each segment) add the segment sums into
+// the wave's interleaved rows rw (row k0, row k0 + 1): detector column
+// floor(u) gets (1 - du) x, floor(u) + 1 gets du x, in two passes so that
+// within one pass every LDS address belongs to one lane; only this wave
+// touches these rows, so the summation order is fixed.
+template <int CLS, bool TAIL>
+__device__ __forceinline__ void col_chunk(float xv, int zi, int nz, double c3, double bxy, int zs, float wm,
+                                          float wn, float2 *rw, int nu, int lane)
+{
+    const bool ok = !TAIL || zi < nz;
+    // the reference's chain: fma(xc3, Z, fma(xc2, Y, fma(xc1, X, xc0))) - 1
+    const double u = fma(c3, (double)(zs + zi), bxy) - 1.0;
+    const double fl = floor(u);
+    const int key = ok ? (int)fl + kKeyBias : 0x40000000 + lane;
+    // tap weights rounded from FP64 (no cancellation in 1 - du)
+    const double dd = u - fl;
+    float ea = (float)(1.0 - dd) * xv, eb = (float)dd * xv;
+    seg_scan_cls<CLS>(ea, eb, key);
+    // next lane's key (wave_shl 1; lane 63 reads 0)
+    const int nkey = dpp_z<0x130>(key);
+    const bool lead = ok && nkey != key;
+    const int bi = key - kKeyBias + 1;  // slot of detector column floor(u) (column -1 -> slot 0)
+    if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi];
+        q.x = fmaf(wm, ea, q.x);
+        q.y = fmaf(wn, ea, q.y);
+        rw[bi] = q;
+    }
+    wave_lds_sync();
+    if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi + 1];
+        q.x = fmaf(wm, eb, q.x);
+        q.y = fmaf(wn, eb, q.y);
+        rw[bi + 1] = q;
+    }
+    wave_lds_sync();
+}
+
+// The strip's columns in order, chunks of 64 planes read in groups of G
+// (G <= Z / 64): the loads of a group are issued together, and the first
+// group of the next column is loaded while the current column is scanned,
+// so the wave waits on global memory once per column at most.
+template <class T, bool UNIT, int CLS, int G>
+__device__ __forceinline__ void col_strip(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
+                                          double c3, float2 *rw, int lane)
+{
+    const auto x = gview(reinterpret_cast<const T *>(ag.x));
+    const int nz = ag.Z, zs = ag.Zs, nu = ag.nu;
+    const int64_t xoff = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy;
+    auto column = [&](int j) {
+        const int2 xy = ag.col_xy[j];
+        return x + (xoff + (int64_t)xy.x * ag.sx + (int64_t)xy.y * ag.sy);
+    };
+    if (j0 >= j1) return;
+    float xn[G > 0 ? G : 1];
+    if constexpr (G > 0) {
+        const auto xcol = column(j0);
+#pragma unroll
+        for (int g = 0; g < G; g++) xn[g] = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)(64 * g + lane));
+    }
+    for (int j = j0; j < j1; j++) {
+        const int2 xy = ag.col_xy[j];
+        const float2 cw = ag.col_w[j];
+        const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
+        const auto xcol = column(j);
+        int zc = 0;
+        if constexpr (G > 0) {
+            // requires Z >= 64 G: the first group is always whole
+            float xg[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) xg[g] = xn[g];
+            if (j + 1 < j1) {
+                const auto xnext = column(j + 1);
+#pragma unroll
+                for (int g = 0; g < G; g++) xn[g] = UNIT ? 1.0f : ldg<T>(xnext, (uint32_t)(64 * g + lane));
+            }
+            while (true) {
+#pragma unroll
+                for (int g = 0; g < G; g++)
+                    col_chunk<CLS, false>(xg[g], zc + 64 * g + lane, nz, c3, bxy, zs, cw.x, cw.y, rw, nu, lane);
+                zc += 64 * G;
+                if (zc + 64 * G > nz) break;
+#pragma unroll
+                for (int g = 0; g < G; g++) xg[g] = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)(zc + 64 * g + lane));
+            }
+        }
+        for (; zc < nz; zc += 64) {
+            const int zi = zc + lane;
+            float xv = 0.0f;
+            if (zi < nz) xv = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)zi);
+            col_chunk<CLS, true>(xv, zi, nz, c3, bxy, zs, cw.x, cw.y, rw, nu, lane);
+        }
+    }
+}
+
+template <class T, bool UNIT, int CLS>
+__device__ __forceinline__ void col_strip_g(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
+                                            double c3, float2 *rw, int lane)
+{
+    if (ag.Z >= 256) col_strip<T, UNIT, CLS, 4>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (ag.Z >= 128) col_strip<T, UNIT, CLS, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (ag.Z >= 64) col_strip<T, UNIT, CLS, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else col_strip<T, UNIT, CLS, 0>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
 }
 
 // Forward projection: one wave per (strip, view); a strip is the set of
@@ -959,10 +1063,10 @@ __device__ __forceinline__ void seg_scan(float &a, float &b, int key, int maxrun
 // coalesced load), form the taps (1 - du) x into detector column floor(u)
 // and du x into floor(u) + 1, and the segmented scan sums the lanes that
 // share a detector column.  The last lane of each segment adds the two
-// sums, weighted (1 - dv) and dv, into the wave's two LDS detector rows:
-// within one instruction every address is touched by one lane and only
-// this wave touches these rows, so the summation order is fixed.  The rows
-// are written out as partial rows pm (row k0) and pn (row k0 + 1);
+// sums, weighted (1 - dv) and dv, into the wave's two LDS detector rows.
+// The scan depth follows from the
+// view's plane step |xc3| and is chosen once per wave.  The rows are
+// written out as partial rows pm (row k0) and pn (row k0 + 1);
 // k_fp_col_res adds them.  UNIT: x taken as 1 (the normaliser).
 constexpr int kColWaves = 4;
 
@@ -991,48 +1095,13 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     // longest run of consecutive planes sharing floor(u): 1/|xc3| + 1
     const double ac3 = fabs(c3);
     const int maxrun = ac3 > 0.0 ? (int)fmin(64.0, floor(1.0 / ac3) + 1.0) : 64;
-    const auto x = gview(reinterpret_cast<const T *>(ag.x));
-    const int nz = ag.Z, zs = ag.Zs;
     const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
-    for (int j = j0; j < j1; j++) {
-        const int2 xy = ag.col_xy[j];
-        const float2 cw = ag.col_w[j];
-        const float wm = cw.x, wn = cw.y;
-        const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
-        const auto xcol = x + ((int64_t)(xy.x - ag.Xs) * ag.sx + (int64_t)(xy.y - ag.Ys) * ag.sy);
-        for (int zc = 0; zc < nz; zc += 64) {
-            const int zi = zc + lane;
-            const bool ok = zi < nz;
-            float xv = 0.0f;
-            if (ok) xv = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)zi);
-            // the reference's chain: fma(xc3, Z, fma(xc2, Y, fma(xc1, X, xc0))) - 1
-            const double u = fma(c3, (double)(zs + zi), bxy) - 1.0;
-            const double fl = floor(u);
-            const int key = ok ? (int)fl + kKeyBias : 0x40000000 + lane;
-            // tap weights rounded from FP64 (no cancellation in 1 - du)
-            const double dd = u - fl;
-            float ea = (float)(1.0 - dd) * xv, eb = (float)dd * xv;
-            seg_scan(ea, eb, key, maxrun);
-            // next lane's key (wave_shl 1; lane 63 reads 0)
-            const int nkey = dpp_z<0x130>(key);
-            const bool lead = ok && nkey != key;
-            const int bi = key - kKeyBias + 1;  // LDS slot of detector column floor(u) (column -1 -> slot 0)
-            if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
-                float2 q = rw[bi];
-                q.x = fmaf(wm, ea, q.x);
-                q.y = fmaf(wn, ea, q.y);
-                rw[bi] = q;
-            }
-            wave_lds_sync();
-            if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
-                float2 q = rw[bi + 1];
-                q.x = fmaf(wm, eb, q.x);
-                q.y = fmaf(wn, eb, q.y);
-                rw[bi + 1] = q;
-            }
-            wave_lds_sync();
-        }
-    }
+    if (maxrun <= 2) col_strip_g<T, UNIT, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (maxrun == 3) col_strip_g<T, UNIT, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (maxrun == 4) col_strip_g<T, UNIT, 3>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (maxrun <= 8) col_strip_g<T, UNIT, 8>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else col_strip_g<T, UNIT, 64>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    wave_lds_sync();
     const int k0 = s - 1;
     const int64_t row = ((int64_t)a * nv + k0) * nu;
     for (int i = lane; i < nu; i += 64) {
