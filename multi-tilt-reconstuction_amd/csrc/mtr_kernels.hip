@@ -1047,6 +1047,161 @@ __device__ __forceinline__ void col_strip(const AgentDev &ag, int j0, int j1, do
     }
 }
 
+// Chunks of 64 R planes for views with |xc3| < 1/R (R = 2, 4): lane l
+// owns planes R l .. R l + R - 1.  Integer crossings of u are then more
+// than R - 1 planes apart, so a lane's planes have at most two keys, and
+// when the lane crosses, the planes after the crossing (its tail) share
+// the key of the next lane's first plane.  Each lane sums its planes in
+// order, hands its tail to the next lane (lane 63 writes its own), and the
+// segmented scan runs over lanes keyed by their first plane, with runs
+// about R times shorter than over single planes.
+template <class T, bool UNIT, int R, int CLS>
+__device__ __forceinline__ void col_chunk_r(const float (&xv)[R], int zc, double c3, double bxy, int zs, float wm,
+                                            float wn, float2 *rw, int nu, int lane, int dt)
+{
+    float A = 0.0f, B = 0.0f, At = 0.0f, Bt = 0.0f;
+    int f0 = 0;
+    bool tail = false;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const double u = fma(c3, (double)(zs + zc + R * lane + r), bxy) - 1.0;
+        const double fl = floor(u);
+        const double dd = u - fl;
+        const float ea = (float)(1.0 - dd) * xv[r], eb = (float)dd * xv[r];
+        if (r == 0) {
+            f0 = (int)fl;
+            A = ea;
+            B = eb;
+        } else if ((int)fl == f0) {
+            A += ea;
+            B += eb;
+        } else {
+            At += ea;
+            Bt += eb;
+            tail = true;
+        }
+    }
+    // the previous lane's tail (wave_shr 1; lane 0 reads 0)
+    A += __int_as_float(dpp_z<0x138>(__float_as_int(At)));
+    B += __int_as_float(dpp_z<0x138>(__float_as_int(Bt)));
+    const int key = f0 + kKeyBias;
+    seg_scan_cls<CLS>(A, B, key);
+    const int nkey = dpp_z<0x130>(key);
+    const bool lead = nkey != key;
+    const int bi = f0 + 1;  // slot of detector column f0
+    const bool own = tail && lane == 63;
+    if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi];
+        q.x = fmaf(wm, A, q.x);
+        q.y = fmaf(wn, A, q.y);
+        rw[bi] = q;
+    }
+    if (own && (unsigned)(bi + dt) <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi + dt];
+        q.x = fmaf(wm, At, q.x);
+        q.y = fmaf(wn, At, q.y);
+        rw[bi + dt] = q;
+    }
+    wave_lds_sync();
+    if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi + 1];
+        q.x = fmaf(wm, B, q.x);
+        q.y = fmaf(wn, B, q.y);
+        rw[bi + 1] = q;
+    }
+    if (own && (unsigned)(bi + 1 + dt) <= (unsigned)(nu + 1)) {
+        float2 q = rw[bi + 1 + dt];
+        q.x = fmaf(wm, Bt, q.x);
+        q.y = fmaf(wn, Bt, q.y);
+        rw[bi + 1 + dt] = q;
+    }
+    wave_lds_sync();
+}
+
+// Strip walk with chunks of 64 R planes read in groups of G chunks
+// (requires Z >= 64 R G), like col_strip; whole chunks beyond the last
+// group are read one at a time, and the planes beyond the last whole chunk
+// go through col_chunk with the Hillis-Steele scan.
+template <class T, bool UNIT, int R, int G, int CLS>
+__device__ __forceinline__ void col_strip_r(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
+                                            double c3, float2 *rw, int lane)
+{
+    const auto x = gview(reinterpret_cast<const T *>(ag.x));
+    const int nz = ag.Z, zs = ag.Zs, nu = ag.nu, dt = c3 > 0.0 ? 1 : -1;
+    const int64_t xoff = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy;
+    auto column = [&](int j) {
+        const int2 xy = ag.col_xy[j];
+        return x + (xoff + (int64_t)xy.x * ag.sx + (int64_t)xy.y * ag.sy);
+    };
+    if (j0 >= j1) return;
+    float xn[G][R];
+    {
+        const auto xcol = column(j0);
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < R; r++) xn[g][r] = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)(64 * R * g + R * lane + r));
+    }
+    for (int j = j0; j < j1; j++) {
+        const int2 xy = ag.col_xy[j];
+        const float2 cw = ag.col_w[j];
+        const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
+        const auto xcol = column(j);
+        float xg[G][R];
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int r = 0; r < R; r++) xg[g][r] = xn[g][r];
+        if (j + 1 < j1) {
+            const auto xnext = column(j + 1);
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    xn[g][r] = UNIT ? 1.0f : ldg<T>(xnext, (uint32_t)(64 * R * g + R * lane + r));
+        }
+        int zc = 0;
+        while (true) {
+#pragma unroll
+            for (int g = 0; g < G; g++)
+                col_chunk_r<T, UNIT, R, CLS>(xg[g], zc + 64 * R * g, c3, bxy, zs, cw.x, cw.y, rw, nu, lane, dt);
+            zc += 64 * R * G;
+            if (zc + 64 * R * G > nz) break;
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    xg[g][r] = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)(zc + 64 * R * g + R * lane + r));
+        }
+        for (; zc + 64 * R <= nz; zc += 64 * R) {
+            float xv[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) xv[r] = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)(zc + R * lane + r));
+            col_chunk_r<T, UNIT, R, CLS>(xv, zc, c3, bxy, zs, cw.x, cw.y, rw, nu, lane, dt);
+        }
+        for (; zc < nz; zc += 64) {
+            const int zi = zc + lane;
+            float xv = 0.0f;
+            if (zi < nz) xv = UNIT ? 1.0f : ldg<T>(xcol, (uint32_t)zi);
+            col_chunk<64, true>(xv, zi, nz, c3, bxy, zs, cw.x, cw.y, rw, nu, lane);
+        }
+    }
+}
+
+// Multi-plane lanes for |xc3| < 1/R with the lane scan class from the
+// longest run of lanes sharing a first key, ceil(maxrun / R).
+template <class T, bool UNIT, int R, int G>
+__device__ __forceinline__ void col_strip_rc(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
+                                             double c3, float2 *rw, int lane, int maxrun)
+{
+    const int m = (maxrun + R - 1) / R;
+    if (m <= 2) col_strip_r<T, UNIT, R, G, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (m == 3) col_strip_r<T, UNIT, R, G, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (m == 4) col_strip_r<T, UNIT, R, G, 3>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else if (m <= 8) col_strip_r<T, UNIT, R, G, 8>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    else col_strip_r<T, UNIT, R, G, 64>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+}
+
 template <class T, bool UNIT, int CLS>
 __device__ __forceinline__ void col_strip_g(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
                                             double c3, float2 *rw, int lane)
@@ -1096,7 +1251,16 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     const double ac3 = fabs(c3);
     const int maxrun = ac3 > 0.0 ? (int)fmin(64.0, floor(1.0 / ac3) + 1.0) : 64;
     const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
-    if (maxrun <= 2) col_strip_g<T, UNIT, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
+    // lanes own R planes where |xc3| < 1/R; about 256 planes are read per group
+    if (ac3 < 0.999 / 4 && ag.Z >= 512) {
+        col_strip_rc<T, UNIT, 4, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane, maxrun);
+    } else if (ac3 < 0.999 / 4 && ag.Z >= 256) {
+        col_strip_rc<T, UNIT, 4, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane, maxrun);
+    } else if (ac3 < 0.999 / 2 && ag.Z >= 256) {
+        col_strip_rc<T, UNIT, 2, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane, maxrun);
+    } else if (ac3 < 0.999 / 2 && ag.Z >= 128) {
+        col_strip_rc<T, UNIT, 2, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane, maxrun);
+    } else if (maxrun <= 2) col_strip_g<T, UNIT, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     else if (maxrun == 3) col_strip_g<T, UNIT, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     else if (maxrun == 4) col_strip_g<T, UNIT, 3>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     else if (maxrun <= 8) col_strip_g<T, UNIT, 8>(ag, j0, j1, c0, c1, c2, c3, rw, lane);

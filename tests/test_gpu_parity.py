@@ -144,6 +144,14 @@ AGENT_GEOMS = GEOMS + [
     ("col135_small_det", dict(phis=[135.0], X=30, Y=30, Z=20, nu=17, nv=13)),  # footprint off the detector
     ("col45_origin", dict(phis=[45.0], origin=(3, -2, 5))),
     ("col75_wide", dict(phis=[75.0], X=150, Y=21, Z=9)),            # several back-projection tiles
+    # deep columns, tilts in 5-degree steps: forward-projection chunks read
+    # in groups of 4 (Z >= 256) and 2 (Z >= 128), lanes owning 4 planes
+    # (|sin theta| < 1/4) and 2 planes (< 1/2) in groups of 1 and 2 chunks,
+    # whole chunks after the last group, and the partial last chunk
+    ("col30_z300", dict(phis=[30.0], X=40, Y=36, Z=300, views=25, nu=360)),
+    ("col45_z256", dict(phis=[45.0], X=24, Y=28, Z=256, views=25, nu=300)),
+    ("col120_z160", dict(phis=[120.0], X=36, Y=40, Z=160, views=25, nu=200)),
+    ("col60_z520", dict(phis=[60.0], X=16, Y=20, Z=520, views=25, nu=560)),
 ]
 
 
