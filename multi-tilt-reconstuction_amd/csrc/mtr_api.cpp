@@ -910,6 +910,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.bpm = bp_mode(plan, xc, yc, ctx->vol, *prj);
     int sW = 0;
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
+    // the streaming FP addresses the volume with 32-bit byte offsets
+    if ((uint64_t)ctx->vol.X * ctx->vol.Y * ctx->vol.Z * elem_size(ctx->precision) > 0xFFFFFFFFull) ag.fp_nc = 0;
     ag.fast = ag.fp_nc ? sW : 0;
     if (!ag.fp_nc && !ag.bpm.sep) {
         ag.col = col_mode(xc, yc, A, *prj, 16);

@@ -89,6 +89,24 @@ __device__ __forceinline__ float ldg<__half>(const __attribute__((address_space(
     return __half2float(__ushort_as_half(b));
 }
 
+// A pointer that is the same in every lane, moved into scalar registers
+// so that loads from it use the scalar-base form with a 32-bit vector
+// offset (no per-load 64-bit address arithmetic).
+__device__ __forceinline__ const __attribute__((address_space(1))) char *uniform_base(const void *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const __attribute__((address_space(1))) char *)(((uint64_t)hi << 32) | lo);
+}
+
+// Element of type T at a byte offset from a scalar base.
+template <class T>
+__device__ __forceinline__ float ldb(const __attribute__((address_space(1))) char *b, uint32_t off)
+{
+    return ldg<T>((const __attribute__((address_space(1))) T *)(b + off), 0u);
+}
+
 // The reference's coordinate chain, left to right with fused multiply-adds.
 __device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
 {
@@ -418,9 +436,9 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const int64_t sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L), sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M),
                   sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W);
     const int64_t base0 = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
-    // uniform base (first plane of the box) + per-lane 32-bit plane offset
-    const auto xb = gview(x + base0 + (int64_t)Wlo * sW);
-    const uint32_t lane_off = (uint32_t)(((lane_ok ? w : Wlo) - Wlo) * sW);
+    // per-lane 32-bit byte offset of the lane's plane (the volume is below
+    // 4 GiB: the host selects this kernel only then)
+    const uint32_t lane_off = (uint32_t)(((lane_ok ? w : Wlo) - Wlo) * sW) * (uint32_t)sizeof(T);
     float acc = 0.f;
     if (i < ag.nu) {
         const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
@@ -440,6 +458,8 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
             const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
             const unsigned mspan = (unsigned)(Mhi - Mlo);
             const int n = (l1 - l0 + 1) * NC;
+            // scalar base: voxel (Llo, Mlo) of the box's first plane
+            const auto xc = uniform_base(x + base0 + (int64_t)Wlo * sW + (int64_t)Llo * sL + (int64_t)Mlo * sM);
             for (int c0 = 0; c0 < n; c0 += 64) {
                 // candidate c of the ray: line l0 + c / NC, point c % NC
                 const int c = c0 + lane;
@@ -450,10 +470,8 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                 // same weight as the per-lane path
                 const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
                 const float wt = ok ? tent(fma((double)j, aM, aM * (mf - mc))) : 0.0f;
-                // offset from the first plane of the box, >= 0
-                const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) : 0;
-                const int64_t lm0 = (int64_t)Llo * sL + (int64_t)Mlo * sM;  // folded into the base below
-                const auto xc = xb + lm0;
+                // byte offset from voxel (Llo, Mlo) of the first plane, >= 0
+                const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) * (int)sizeof(T) : 0;
                 // compact the nonzero taps of this chunk into the wave's LDS
                 // list (order preserved), padded to a multiple of 8
                 const uint64_t mask = __ballot(wt > 0.0f);
@@ -466,16 +484,38 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (int e = 0; e < cnt; e += 8) {
-                    float xv[8], ww[8];
+                // taps in groups of 8 held in two register sets: the next
+                // group's loads are in flight while the current one is summed
+                const int cnt8 = (cnt + 7) & ~7;
+                float xa[8], wa[8], xb[8], wb[8];
+                auto fetch = [&](int e, float(&xv)[8], float(&ww)[8]) {
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         const int2 tp = taps[wid][e + q];
                         ww[q] = __int_as_float(tp.y);
-                        xv[q] = ldg<T>(xc, (uint32_t)tp.x + lane_off);
+                        xv[q] = ldb<T>(xc, (uint32_t)tp.x + lane_off);
                     }
+                };
+                auto sum = [&](const float(&xv)[8], const float(&ww)[8]) {
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc = fmaf(ww[q], xv[q], acc);
+                };
+                if (cnt > 0) {
+                    fetch(0, xa, wa);
+                    int e = 0;
+                    for (; e + 16 < cnt8; e += 16) {
+                        fetch(e + 8, xb, wb);
+                        sum(xa, wa);
+                        fetch(e + 16, xa, wa);
+                        sum(xb, wb);
+                    }
+                    if (e + 8 < cnt8) {
+                        fetch(e + 8, xb, wb);
+                        sum(xa, wa);
+                        sum(xb, wb);
+                    } else {
+                        sum(xa, wa);
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();

@@ -146,12 +146,14 @@ AGENT_GEOMS = GEOMS + [
     ("col75_wide", dict(phis=[75.0], X=150, Y=21, Z=9)),            # several back-projection tiles
     # deep columns, tilts in 5-degree steps: forward-projection chunks read
     # in groups of 4 (Z >= 256) and 2 (Z >= 128), lanes owning 4 planes
-    # (|sin theta| < 1/4) and 2 planes (< 1/2) in groups of 1 and 2 chunks,
-    # whole chunks after the last group, and the partial last chunk
+    # (|sin theta| < 1/4) and 2 planes (< 1/2, and > 1/2 without a scan) in
+    # groups of 1 and 2 chunks, whole chunks after the last group, and the
+    # partial last chunk
     ("col30_z300", dict(phis=[30.0], X=40, Y=36, Z=300, views=25, nu=360)),
     ("col45_z256", dict(phis=[45.0], X=24, Y=28, Z=256, views=25, nu=300)),
     ("col120_z160", dict(phis=[120.0], X=36, Y=40, Z=160, views=25, nu=200)),
     ("col60_z520", dict(phis=[60.0], X=16, Y=20, Z=520, views=25, nu=560)),
+    ("col135_z400", dict(phis=[135.0], X=20, Y=16, Z=400, views=25, nu=440)),
 ]
 
 
