@@ -94,3 +94,24 @@ def test_split_precision_is_deterministic(mtr, reference, name):
     a = solve(mtr, pb, precs(mtr)[name], M=2, S=2)
     b = solve(mtr, pb, precs(mtr)[name], M=2, S=2)
     assert np.array_equal(a[0], b[0]) and a[2] == b[2] and a[3] == b[3]
+
+
+@pytest.mark.parametrize("name", ["fp32", "bf16", "fp16"])
+def test_deep_column_agent_against_fp64_oracle(mtr, orc, name):
+    # oblique axis with 300-plane columns and tilts in 5-degree steps: the
+    # column-plan forward projection's multi-plane lanes and grouped reads
+    # in every storage type (BPT, then 2 SIRT iterations)
+    pb = mtr.synth.make_problem("deep", 24, 20, 300, 6, [45.0], seed=23, amin=2, amax=6,
+                                theta=np.linspace(-60, 60, 25), nu=330)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    x0 = orc.bpt_f64(prj, vol, xc, yc, p)
+    ref = orc.sirt_f64(prj, vol, xc, yc, p, x0, 2, 0.5)
+    with mtr.Context(0, precs(mtr)[name]) as c:
+        c.set_volume(vol)
+        c.add_agent(prj, xc, yc, p)
+        assert c.agent_kernels(0) == (mtr.FPK_COLUMN, mtr.BPK_COLUMN)
+        c.bpt(0, out=False)
+        got = c.sirt(0, 2, 0.5)
+    err = orc.rel_l2(got, ref)
+    say(f"{name}: deep column SIRT rel_l2 vs FP64 oracle = {err:.3e} (bound {BOUND[name]:.0e})")
+    assert err < BOUND[name]
