@@ -1271,14 +1271,17 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     extern __shared__ float col_rows[];
     const AgentDev &ag = ags[blockIdx.y];
     const int nvg = (ag.A + kColWaves - 1) / kColWaves;
-    // XCD-aware order, strip-major: the view groups of one strip run on
-    // one XCD back to back and share the strip's columns in its L2
-    const int nb = ag.nstrip * nvg, per = (nb + 7) / 8;
-    if (blockIdx.x >= (unsigned)(per * 8)) return;
-    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-    if (b >= nb) return;
+    // XCD-aware order: strips are dealt to the 8 XCDs in turn (strip
+    // lengths vary smoothly across the volume, so contiguous ranges would
+    // load the XCDs unevenly), and the view groups of one strip run on one
+    // XCD back to back, sharing the strip's columns in its L2
+    const int spx = (ag.nstrip + 7) / 8;  // strips per XCD
+    const int idx = blockIdx.x >> 3, sx = idx / nvg;
+    if (sx >= spx) return;
+    const int s = sx * 8 + (blockIdx.x & 7);
+    if (s >= ag.nstrip) return;
     const int lane = threadIdx.x & 63, wid = wave_id();
-    const int s = b / nvg, a = (b - s * nvg) * kColWaves + wid;
+    const int a = (idx - sx * nvg) * kColWaves + wid;
     if (a >= ag.A) return;  // no block-wide barrier below
     const int nu = ag.nu, nv = ag.nv, L = nu + 2;
     // the wave's two detector rows, interleaved: (row k0, row k0 + 1)
@@ -2232,8 +2235,7 @@ hipError_t launch_permute(int prec, const void *src, void *dst, int nX, int nY, 
 
 int fp_col_blocks(const AgentDev &ag)
 {
-    const int nb = ag.nstrip * ((ag.A + kColWaves - 1) / kColWaves);
-    return ((nb + 7) / 8) * 8;
+    return 8 * ((ag.nstrip + 7) / 8) * ((ag.A + kColWaves - 1) / kColWaves);
 }
 
 int fp_col_max_nu() { return (int)(65536 / (kColWaves * 2 * sizeof(float))) - 2; }
