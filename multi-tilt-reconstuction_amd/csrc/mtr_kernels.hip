@@ -1242,6 +1242,8 @@ __device__ __forceinline__ void col_strip_rc(const AgentDev &ag, int j0, int j1,
     else col_strip_r<T, UNIT, R, G, 64>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
 }
 
+// One plane per lane: chunk groups of 4, 2 or 1 by the volume depth (none
+// below 64 planes: partial chunks only).
 template <class T, bool UNIT, int CLS>
 __device__ __forceinline__ void col_strip_g(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
                                             double c3, float2 *rw, int lane)
@@ -1254,15 +1256,16 @@ __device__ __forceinline__ void col_strip_g(const AgentDev &ag, int j0, int j1, 
 
 // Forward projection: one wave per (strip, view); a strip is the set of
 // columns sharing k0.  The wave walks the strip's columns in a fixed
-// order; for each column its 64 lanes take 64 consecutive planes (one
-// coalesced load), form the taps (1 - du) x into detector column floor(u)
-// and du x into floor(u) + 1, and the segmented scan sums the lanes that
-// share a detector column.  The last lane of each segment adds the two
-// sums, weighted (1 - dv) and dv, into the wave's two LDS detector rows.
-// The scan depth follows from the
-// view's plane step |xc3| and is chosen once per wave.  The rows are
-// written out as partial rows pm (row k0) and pn (row k0 + 1);
-// k_fp_col_res adds them.  UNIT: x taken as 1 (the normaliser).
+// order; for each column its lanes take consecutive planes (one coalesced
+// load per 64 planes), form the taps (1 - du) x into detector column
+// floor(u) and du x into floor(u) + 1, and the segmented scan sums the
+// lanes that share a detector column.  The last lane of each segment adds
+// the two sums, weighted (1 - dv) and dv, into the wave's two LDS detector
+// rows.  The view's plane step |xc3| fixes, once per wave, the planes per
+// lane (1, 2 or 4) and the scan depth; the volume depth fixes how many
+// chunks are read at a time.  The rows are written out as partial rows pm
+// (row k0) and pn (row k0 + 1); k_fp_col_res adds them.  UNIT: x taken as
+// 1 (the normaliser).
 constexpr int kColWaves = 4;
 
 template <class T, bool UNIT>
