@@ -355,3 +355,20 @@ def test_slab_fp_sums_to_full(mtr, orc):
         acc_c += cc
     np.testing.assert_allclose(acc_s, s, rtol=1e-5, atol=1e-5)
     np.testing.assert_allclose(acc_c, c, rtol=1e-5, atol=1e-5)
+
+
+def test_oracle_runs_clean_under_asan_and_ubsan():
+    """Every oracle function on tiny ragged problems (offset boxes,
+    detectors narrower than the box, tilted and general affine views)
+    built with AddressSanitizer and UndefinedBehaviorSanitizer: no
+    out-of-bounds access, overflow or other undefined behaviour (the check
+    aborts on the first report)."""
+    import subprocess
+
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    b = subprocess.run(["make", "-s", "-C", here, "sanitize"], capture_output=True, text=True, timeout=300)
+    assert b.returncode == 0, b.stdout + b.stderr
+    r = subprocess.run([os.path.join(here, "_san", "sanitize_check")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "runtime error" not in r.stderr and "Sanitizer" not in r.stderr
+    assert "12 of 12 cases clean" in r.stdout
