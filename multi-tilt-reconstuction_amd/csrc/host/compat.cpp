@@ -82,7 +82,8 @@ void helpInfo()
         "  --sirt-iters N  SIRT iterations per MACE step (default 10)\n"
         "  --precision P   fp32 | bf16 | fp16 storage (default fp32)\n"
         "  --form F        fast | plain (default fast)\n"
-        "  --device D      GPU index (default 0)\n");
+        "  --device D      GPU index (default 0)\n"
+        "  --tune-cache F  keep the autotuner's tile choices in file F across runs\n");
 }
 
 int readOptions(int argc, char **argv, Options *opts)
@@ -98,6 +99,7 @@ int readOptions(int argc, char **argv, Options *opts)
         {"precision", required_argument, nullptr, 1002},
         {"form", required_argument, nullptr, 1003},
         {"device", required_argument, nullptr, 1004},
+        {"tune-cache", required_argument, nullptr, 1005},
         {nullptr, 0, nullptr, 0},
     };
     optind = 1;
@@ -126,6 +128,8 @@ int readOptions(int argc, char **argv, Options *opts)
                 else { std::fprintf(stderr, "unknown form '%s'\n", optarg); return -1; }
                 break;
             case 1004: opts->device = std::atoi(optarg); break;
+            // the library reads and extends the cache named by MTR_TUNE_CACHE
+            case 1005: setenv("MTR_TUNE_CACHE", optarg, 1); break;
             case 'i': {
                 // K,mrc_1..mrc_K,txbr_1..txbr_K (ReadAndInitialOptions.cpp:104-125)
                 std::stringstream ss(optarg);

@@ -141,15 +141,17 @@ def test_cli_reconstruction_matches_oracle(mtr, orc, tmp_path):
     pb = mtr.synth.make_problem("cli", 28, 24, 10, 6, [0.0, 90.0], seed=21, amin=2, amax=6,
                                 theta=np.linspace(-60, 60, 9))
     sts, txs = write_problem(mtr, pb, tmp_path)
-    out = str(tmp_path / "out.mrc")
+    out, tune = str(tmp_path / "out.mrc"), tmp_path / "tune.txt"
     print("[gpu] cli run", flush=True)
     r = subprocess.run([CLI, "-i", ",".join([str(pb.K)] + sts + txs), "-o", out, "-n", "2", "-s", "0.5", "-m", "0.5",
-                        "--sirt-iters", "3", "-x", "8", "-y", "8", "-z", "8"], capture_output=True, text=True,
-                       timeout=120)
+                        "--sirt-iters", "3", "-x", "8", "-y", "8", "-z", "8", "--tune-cache", str(tune)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = mtr.mrc.read(out)
     ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 2, 3, 0.5, 0.5)
     assert orc.rel_l2(got, ref) < 1e-5
+    # the autotuner kept one tile choice per agent geometry in the cache file
+    assert tune.exists() and len(tune.read_text().splitlines()) >= 1
 
 
 @pytest.mark.gpu
