@@ -130,12 +130,13 @@ int mtr_ctx_set_volume(mtr_ctx *ctx, const mtr_volume_size *vol);
 /* ---- agents (one tilt series each) ---------------------------------- */
 /* Uploads projections p[AngN][Y][X] and geometry once; they stay resident.
  * Allocates the agent's working volume, MACE state, residual and
- * normaliser. */
+ * normaliser.  MTR_ERR_NONFINITE for a NaN or Inf in p or the geometry. */
 int mtr_agent_add(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc,
                   const double *yc, const float *p_host, int *agent_id);
 /* Same, with p already in device memory (not copied; must stay valid
  * until the context is destroyed).  p_dev is float for every precision
- * and is converted into the agent's own storage. */
+ * and is converted into the agent's own storage; it is not scanned for
+ * non-finite values (the geometry is). */
 int mtr_agent_add_device(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc,
                          const double *yc, const float *p_dev, int *agent_id);
 int mtr_agent_count(const mtr_ctx *ctx, int *count);
@@ -263,7 +264,9 @@ int mtr_server_destroy(mtr_server *srv);
 /* Device bytes a request of this shape holds while active (upper bound). */
 int mtr_server_footprint(const mtr_server *srv, const mtr_volume_size *vol, int K,
                          const mtr_projection_size *prj, size_t *bytes);
-/* MTR_ERR_OOM when the request alone exceeds the budget. */
+/* MTR_ERR_OOM when the request alone exceeds the budget; MTR_ERR_NONFINITE
+ * for a NaN or Inf in its geometry or projections (checked here, so a
+ * queued request always runs). */
 int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const mtr_projection_size *prj,
                       const double *const *xc, const double *const *yc, const float *const *p, int M,
                       int S, float lambda, float rho, int64_t *request_id);

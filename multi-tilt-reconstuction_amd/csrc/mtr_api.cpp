@@ -58,6 +58,23 @@ bool vol_valid(const mtr_volume_size *v)
            v->Yend - v->Ystart == v->Y && v->Zend - v->Zstart == v->Z;
 }
 
+// Index of the first NaN or Inf in a[0..n) or -1: an all-ones exponent,
+// tested on the bits so that the scan vectorises.
+int64_t first_nonfinite(const float *a, int64_t n)
+{
+    const uint32_t *u = reinterpret_cast<const uint32_t *>(a);
+    constexpr int64_t kBlock = 1 << 16;
+    for (int64_t b = 0; b < n; b += kBlock) {
+        const int64_t e = std::min(n, b + kBlock);
+        uint32_t bad = 0;
+        for (int64_t i = b; i < e; i++) bad |= (u[i] & 0x7f800000u) == 0x7f800000u;
+        if (bad)
+            for (int64_t i = b; i < e; i++)
+                if ((u[i] & 0x7f800000u) == 0x7f800000u) return i;
+    }
+    return -1;
+}
+
 bool prj_valid(const mtr_projection_size *p)
 {
     return p && p->X > 0 && p->Y > 0 && p->AngN > 0 && p->AngN <= 100000;
@@ -867,14 +884,23 @@ int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda
     return MTR_OK;
 }
 
+// p_checked: p_host was already scanned for non-finite values (a request
+// the server validated at submission).
 int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc, const double *yc,
-                     const float *p_host, const float *p_dev, int *agent_id)
+                     const float *p_host, const float *p_dev, int *agent_id, bool p_checked = false)
 {
     if (!ctx || !prj || !xc || !yc || (!p_host && !p_dev)) return fail(MTR_ERR_ARG, "null argument");
     if (!ctx->vol_set) return fail(MTR_ERR_STATE, "mtr_ctx_set_volume must precede mtr_agent_add");
     if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size %d x %d x %d", prj->X, prj->Y, prj->AngN);
     if ((int)ctx->agents.size() >= kMaxAgents)
         return fail(MTR_ERR_RANGE, "at most %d agents per context", kMaxAgents);
+    if (p_host && !p_checked) {
+        // a NaN or Inf pixel would reach every voxel its views touch
+        const int64_t i = first_nonfinite(p_host, (int64_t)prj->X * prj->Y * prj->AngN);
+        if (i >= 0)
+            return fail(MTR_ERR_NONFINITE, "projection value %lld (view %lld) is not finite", (long long)i,
+                        (long long)(i / ((int64_t)prj->X * prj->Y)));
+    }
     MTR_TRY(set_device(ctx));
     const int A = prj->AngN;
     std::vector<ViewCoef> coef(A);
@@ -1913,7 +1939,8 @@ int admit(mtr_server *srv, ServerReq *r)
     MTR_TRY(mtr_ctx_set_volume(c, &r->vol));
     for (size_t a = 0; a < r->prj.size(); a++) {
         int id = 0;
-        MTR_TRY(mtr_agent_add(c, &r->prj[a], r->xc[a].data(), r->yc[a].data(), r->p[a].data(), &id));
+        MTR_TRY(agent_add_common(c, &r->prj[a], r->xc[a].data(), r->yc[a].data(), r->p[a].data(), nullptr, &id,
+                                 true));
         std::vector<float>().swap(r->p[a]);  // resident now
     }
     for (int a = 0; a < (int)c->agents.size(); a++) MTR_TRY(ensure_normaliser(c, a));
@@ -1987,6 +2014,13 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
     for (int a = 0; a < K; a++) {
         if (!xc[a] || !yc[a] || !p[a]) return fail(MTR_ERR_ARG, "null series %d", a);
         const size_t A = (size_t)prj[a].AngN, P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
+        // rejected here, not at admission: a queued request always runs
+        for (size_t i = 0; i < 4 * A; i++)
+            if (!std::isfinite(xc[a][i]) || !std::isfinite(yc[a][i]))
+                return fail(MTR_ERR_NONFINITE, "series %d: geometry coefficient %zu is not finite", a, i);
+        const int64_t bad = first_nonfinite(p[a], (int64_t)P);
+        if (bad >= 0)
+            return fail(MTR_ERR_NONFINITE, "series %d: projection value %lld is not finite", a, (long long)bad);
         r->xc.emplace_back(xc[a], xc[a] + 4 * A);
         r->yc.emplace_back(yc[a], yc[a] + 4 * A);
         r->p.emplace_back(p[a], p[a] + P);

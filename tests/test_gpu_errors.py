@@ -46,7 +46,7 @@ def test_call_order_and_index_errors(mtr, orc):
         assert orc.rel_l2(xb, orc.bpt(prj, pb.vol, xc, yc, p)) < 2e-6
 
 
-def test_bad_shapes_and_nonfinite_geometry(mtr):
+def test_bad_shapes_and_nonfinite_inputs(mtr):
     pb = small(mtr)
     prj, xc, yc, p = pb.prjs[0], pb.xcs[0], pb.ycs[0], pb.ps[0]
     with mtr.Context(0) as c:
@@ -57,6 +57,26 @@ def test_bad_shapes_and_nonfinite_geometry(mtr):
             bx, by = xc.copy(), yc.copy()
             (bx if which == "xc" else by)[idx] = val
             assert code_of(lambda: c.add_agent(prj, bx, by, p)) == ERR_NONFINITE
+        bad = np.array(p, np.float32).reshape(-1)
+        bad[prj.X * prj.Y * 3 + 7] = np.nan
+        assert code_of(lambda: c.add_agent(prj, xc, yc, bad)) == ERR_NONFINITE
         # rejected agents leave nothing behind; a good one is agent 0
         assert c.add_agent(prj, xc, yc, p) == 0
         assert np.isfinite(c.bpt(0)).all()
+
+
+def test_server_rejects_nonfinite_requests_at_submission(mtr):
+    pb = small(mtr)
+    bad = np.array(pb.ps[0], np.float32).reshape(-1)
+    bad[5] = np.inf
+    badx = pb.xcs[0].copy()
+    badx[2] = np.nan
+    with mtr.Server(0) as srv:
+        assert code_of(lambda: srv.submit(pb.vol, pb.prjs, pb.xcs, pb.ycs, [bad], 1, 1, 0.5, 0.5)) == ERR_NONFINITE
+        assert code_of(lambda: srv.submit(pb.vol, pb.prjs, [badx], pb.ycs, pb.ps, 1, 1, 0.5, 0.5)) == ERR_NONFINITE
+        assert srv.status()["pending"] == 0
+        rid = srv.submit(pb.vol, pb.prjs, pb.xcs, pb.ycs, pb.ps, 1, 1, 0.5, 0.5)
+        while srv.status()["done"] < 1:
+            srv.step()
+        out, _ = srv.result(rid)
+        assert np.isfinite(out).all()
