@@ -184,6 +184,18 @@ int mtr_mace_init(mtr_ctx *ctx);
  * S SIRT iterations per agent.  wbar_host (nullable) receives w-bar. */
 int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form,
                    float *wbar_host);
+/* Convergence control and checkpoint/resume (no counterpart in the
+ * reference, which always runs -n steps and keeps no state between runs).
+ * Continues a fast MACE solve from the current state w_a, w-bar: after
+ * mtr_mace_init or a solve, or after mtr_set_volume of MTR_VOL_W for every
+ * agent and of MTR_VOL_WBAR (resume from a checkpoint taken with
+ * mtr_get_volume).  Runs at most M steps; with rel_tol > 0 it stops after
+ * the first step whose FP64 consensus difference sum_a ||w_a - wbar||^2
+ * changed by at most rel_tol times the previous step's value.  Continuing
+ * is bit-identical to not having stopped.  steps_run and wbar_host are
+ * optional. */
+int mtr_mace_continue(mtr_ctx *ctx, int M, int S, float lambda, float rho, double rel_tol,
+                      int *steps_run, float *wbar_host);
 
 /* ---- state access ---------------------------------------------------- */
 int mtr_get_volume(mtr_ctx *ctx, int which, int agent, float *host);

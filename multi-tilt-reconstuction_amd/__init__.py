@@ -201,6 +201,16 @@ class Context:
         )
         return o
 
+    def mace_continue(self, M: int, S: int, lam: float, rho: float, rel_tol: float = 0.0, out: bool = True):
+        """At most M more fast MACE steps from the current state (after mace_init, a solve, or a checkpoint
+        restored with set_volume_data); stops early once the FP64 consensus difference changes by at most
+        rel_tol of its previous value.  Returns (w-bar or None, steps run)."""
+        o = _np.empty(self.shape, _np.float32) if out else None
+        n = _ct.c_int()
+        _check(lib.mtr_mace_continue(self._h, M, S, _ct.c_float(lam), _ct.c_float(rho), _ct.c_double(rel_tol),
+                                     _ct.byref(n), _fp(o) if out else None), "mace_continue")
+        return o, n.value
+
     def get_volume(self, which: int, agent: int = 0) -> _np.ndarray:
         o = _np.empty(self.shape, _np.float32)
         _check(lib.mtr_get_volume(self._h, which, agent, _fp(o)), "get_volume")

@@ -1530,6 +1530,56 @@ static int step_graph(mtr_ctx *ctx, int S, float lambda, float rho)
     return MTR_OK;
 }
 
+// The FP64 consensus difference of the last step (mtr_consensus_diff2).
+static int consensus_d2(mtr_ctx *ctx, double *out)
+{
+    HIP_TRY(launch_reduce(ctx->d_cons_partial, ctx->cons_blocks, ctx->d_cons_scalar, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(out, ctx->d_cons_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
+}
+
+int mtr_mace_continue(mtr_ctx *ctx, int M, int S, float lambda, float rho, double rel_tol, int *steps_run,
+                      float *wbar_host)
+{
+    if (!ctx) return fail(MTR_ERR_ARG, "null context");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    if (M < 0 || S < 0) return fail(MTR_ERR_ARG, "negative iteration count");
+    if (!(rel_tol >= 0.0)) return fail(MTR_ERR_ARG, "rel_tol must be >= 0");
+    MTR_TRY(set_device(ctx));
+    for (int a = 0; a < (int)ctx->agents.size(); a++) MTR_TRY(ensure_normaliser(ctx, a));
+    MTR_TRY(tune_all(ctx));
+    // working volumes from the state, z_a = 2 w-bar - w_a: the formula and
+    // rounding a consensus step uses for the next step's input
+    for (auto &ag : ctx->agents) {
+        {
+            Timed t(ctx, MTR_K_ELEMENTWISE);
+            HIP_TRY(launch_form_z(ctx->precision, ctx->N, ag.d_w, ctx->d_wbar, work_canon(ag), ctx->stream));
+        }
+        MTR_TRY(from_canon(ctx, ag));
+    }
+    const int cb = consensus_blocks(ctx->N);
+    const bool graph = ctx->use_graphs && !ctx->timing && M > 0;
+    if (graph) MTR_TRY(step_graph(ctx, S, lambda, rho));
+    int m = 0;
+    double prev = -1.0;
+    while (m < M) {
+        if (graph) HIP_TRY(hipGraphLaunch(ctx->step_graph, ctx->stream));
+        else MTR_TRY(mace_step(ctx, S, lambda, rho));
+        m++;
+        ctx->cons_blocks = cb;
+        if (rel_tol > 0.0) {
+            double d2 = 0.0;
+            MTR_TRY(consensus_d2(ctx, &d2));
+            if (prev >= 0.0 && std::fabs(d2 - prev) <= rel_tol * prev) break;
+            prev = d2;
+        }
+    }
+    if (steps_run) *steps_run = m;
+    if (wbar_host) MTR_TRY(copy_volume_out(ctx, ctx->d_wbar, wbar_host));
+    return collect_timing(ctx);
+}
+
 int mtr_mace_solve(mtr_ctx *ctx, int M, int S, float lambda, float rho, int form, float *wbar_host)
 {
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
@@ -1641,9 +1691,7 @@ int mtr_consensus_diff2(mtr_ctx *ctx, double *out)
     if (!ctx || !out) return fail(MTR_ERR_ARG, "null argument");
     if (ctx->cons_blocks <= 0) return fail(MTR_ERR_STATE, "no consensus step has run");
     MTR_TRY(set_device(ctx));
-    HIP_TRY(launch_reduce(ctx->d_cons_partial, ctx->cons_blocks, ctx->d_cons_scalar, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(out, ctx->d_cons_scalar, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    MTR_TRY(consensus_d2(ctx, out));
     return MTR_OK;
 }
 

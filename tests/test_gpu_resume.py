@@ -1,0 +1,137 @@
+"""Convergence control and checkpoint/resume of the fast MACE solve.
+
+The reference always runs its -n MACE steps in one process
+(MultiAxis.cpp:40-71) and keeps no state between runs; `mtr_mace_continue`
+adds the two controls a long solve needs, on the same step kernels:
+
+* continuing from the state (w_a, w-bar) is bit-identical to a solve that
+  never stopped, in the same context or in a new one restored from a
+  checkpoint taken with get_volume, in every storage precision;
+* with rel_tol > 0 the loop stops after the first step whose FP64 consensus
+  difference sum_a ||w_a - w-bar||^2 changed by at most rel_tol of the
+  previous step's value, and the result is that of a solve of that many steps.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ERR_ARG, ERR_STATE = -1, -6
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def problem(mtr, seed=31):
+    # a row agent, a column agent and a separable agent
+    return mtr.synth.make_problem("resume", 34, 30, 14, 6, [0.0, 45.0, 90.0], seed=seed, amin=2, amax=6,
+                                  theta=np.linspace(-60, 60, 9))
+
+
+def load(mtr, pb, precision=None):
+    c = mtr.Context(0) if precision is None else mtr.Context(0, precision)
+    c.set_volume(pb.vol)
+    for a in range(pb.K):
+        c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+    return c
+
+
+PRECS = [("fp32", 0), ("bf16", 1), ("fp16", 2)]
+
+
+@pytest.mark.parametrize("name,prec", PRECS, ids=[p[0] for p in PRECS])
+def test_continue_in_place_equals_one_solve(mtr, name, prec):
+    pb = problem(mtr)
+    say("continue in place", name)
+    with load(mtr, pb, prec) as c:
+        whole = c.mace_solve(4, 3, 0.5, 0.5)
+        d_whole = c.consensus_diff2()
+    with load(mtr, pb, prec) as c:
+        c.mace_init()
+        _, n1 = c.mace_continue(1, 3, 0.5, 0.5)
+        _, n2 = c.mace_continue(3, 3, 0.5, 0.5)
+        parts = c.get_volume(mtr.VOL_WBAR)
+        assert (n1, n2) == (1, 3)
+        assert c.consensus_diff2() == d_whole
+    assert np.array_equal(whole, parts)
+
+
+@pytest.mark.parametrize("name,prec", PRECS, ids=[p[0] for p in PRECS])
+def test_resume_from_checkpoint_in_a_new_context(mtr, name, prec):
+    pb = problem(mtr, seed=32)
+    say("resume from checkpoint", name)
+    with load(mtr, pb, prec) as c:
+        whole = c.mace_solve(3, 2, 0.6, 0.4)
+    with load(mtr, pb, prec) as c:
+        c.mace_solve(1, 2, 0.6, 0.4, out=False)
+        ckpt = [c.get_volume(mtr.VOL_W, a) for a in range(pb.K)] + [c.get_volume(mtr.VOL_WBAR)]
+    with load(mtr, pb, prec) as c:
+        for a in range(pb.K):
+            c.set_volume_data(mtr.VOL_W, a, ckpt[a])
+        c.set_volume_data(mtr.VOL_WBAR, 0, ckpt[-1])
+        resumed, n = c.mace_continue(2, 2, 0.6, 0.4)
+    assert n == 2
+    assert np.array_equal(whole, resumed)
+
+
+def test_zero_steps_leave_the_state(mtr):
+    pb = problem(mtr, seed=33)
+    with load(mtr, pb) as c:
+        w1 = c.mace_solve(1, 2, 0.5, 0.5)
+        out, n = c.mace_continue(0, 2, 0.5, 0.5)
+        assert n == 0 and np.array_equal(out, w1)
+
+
+def test_early_stop_follows_the_consensus_difference(mtr):
+    pb = problem(mtr, seed=34)
+    M, S, lam, rho = 12, 2, 0.5, 0.5
+    say("early stop trajectory")
+    # the trajectory of the consensus difference, one step per call
+    d = []
+    with load(mtr, pb) as c:
+        c.mace_init()
+        for _ in range(M):
+            c.mace_continue(1, S, lam, rho, out=False)
+            d.append(c.consensus_diff2())
+    assert all(np.isfinite(d)) and d[-1] < d[0]
+    changes = [abs(d[m] - d[m - 1]) / d[m - 1] for m in range(1, M)]
+    # a tolerance that stops strictly inside the run
+    tol = float(np.median(changes))
+    expect = next(m + 1 for m in range(1, M) if abs(d[m] - d[m - 1]) <= tol * d[m - 1])
+    assert 2 <= expect < M
+    say("early stop at", expect, "of", M, "tol", tol)
+    with load(mtr, pb) as c:
+        c.mace_init()
+        stopped, n = c.mace_continue(M, S, lam, rho, rel_tol=tol)
+        assert c.consensus_diff2() == d[n - 1]
+    assert n == expect
+    with load(mtr, pb) as c:
+        assert np.array_equal(stopped, c.mace_solve(n, S, lam, rho))
+    # rel_tol 0 never stops early
+    with load(mtr, pb) as c:
+        c.mace_init()
+        assert c.mace_continue(M, S, lam, rho, rel_tol=0.0, out=False)[1] == M
+
+
+def test_continue_errors(mtr):
+    pb = problem(mtr, seed=35)
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        for call in (lambda: c.mace_continue(1, 1, 0.5, 0.5),):
+            with pytest.raises(Exception) as ei:
+                call()
+            assert ei.value.code == ERR_STATE
+        c.add_agent(pb.prjs[0], pb.xcs[0], pb.ycs[0], pb.ps[0])
+        for bad in (dict(M=-1), dict(S=-1), dict(rel_tol=-1e-3), dict(rel_tol=float("nan"))):
+            kw = dict(M=1, S=1, lam=0.5, rho=0.5, rel_tol=0.0)
+            kw.update(bad)
+            with pytest.raises(Exception) as ei:
+                c.mace_continue(**kw)
+            assert ei.value.code == ERR_ARG, bad
+        # without init the state starts at zero: a solve from the zero volume
+        out, n = c.mace_continue(1, 1, 0.5, 0.5)
+        assert n == 1 and np.isfinite(out).all()
