@@ -200,6 +200,17 @@ int mtr_mace_continue(mtr_ctx *ctx, int M, int S, float lambda, float rho, doubl
 /* ---- state access ---------------------------------------------------- */
 int mtr_get_volume(mtr_ctx *ctx, int which, int agent, float *host);
 int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host);
+/* MACE checkpoint files (no counterpart in the reference, which keeps no
+ * state between runs): save writes the state w_0 .. w_{K-1}, w-bar as
+ * float32 volumes after a 64-byte header (magic "MTRCKPT1", K, the box, the
+ * caller's count of steps done, the storage precision, a hash of every
+ * agent's detector size and coefficients); load checks magic, agent count,
+ * box, geometry hash and file length (MTR_ERR_SHAPE), and NaN or Inf
+ * (MTR_ERR_NONFINITE) before it replaces the state, then returns the step
+ * count.  mtr_mace_continue resumes from a loaded state bit-exactly when
+ * the precision is the one the checkpoint was written in. */
+int mtr_checkpoint_save(mtr_ctx *ctx, const char *path, int steps_done);
+int mtr_checkpoint_load(mtr_ctx *ctx, const char *path, int *steps_done);
 /* Device pointer of w-bar (FP32 storage only; for device-side consumers). */
 int mtr_wbar_device(mtr_ctx *ctx, void **dev_ptr);
 

@@ -5,7 +5,8 @@
  * Same names, argument meaning and call order as Autumn1998/Multi-tilt-
  * Reconstuction, so its driver (MultiAxis.cpp) links against this header
  * with the MPI plumbing removed:
- *   struct Options            ReadAndInitialOptions.h:25-43 (+ sirt_iters, precision, form, device)
+ *   struct Options            ReadAndInitialOptions.h:25-43 (+ sirt_iters, precision, form, device,
+ *                             tol, checkpoint, resume)
  *   readOptions               ReadAndInitialOptions.cpp:127-172 (same short flags)
  *   struct InputData          MrcFileIO.h:13-29
  *   struct OutputModel        MrcFileIO.h:31-45
@@ -58,6 +59,10 @@ struct Options {
     int precision = MTR_PREC_FP32;
     int form = MTR_FORM_FAST;
     int device = 0;
+    double tol = 0.0;           // --tol: stop once the consensus difference changes by <= tol (relative)
+    std::string checkpoint;     // --checkpoint: write the MACE state here after the solve
+    std::string resume;         // --resume: continue from this checkpoint instead of the BPT initial model
+    int steps_done = 0;         // out: MACE steps done in total (resumed ones included)
     bool help = false;
 };
 

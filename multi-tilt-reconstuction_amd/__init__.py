@@ -22,6 +22,7 @@ Reference correspondence (Autumn1998/Multi-tilt-Reconstuction):
 from __future__ import annotations
 
 import ctypes as _ct
+import os as _os
 
 import numpy as _np
 
@@ -210,6 +211,16 @@ class Context:
         _check(lib.mtr_mace_continue(self._h, M, S, _ct.c_float(lam), _ct.c_float(rho), _ct.c_double(rel_tol),
                                      _ct.byref(n), _fp(o) if out else None), "mace_continue")
         return o, n.value
+
+    def checkpoint_save(self, path: str, steps_done: int) -> None:
+        """Writes the MACE state (every w_a, then w-bar) with the caller's count of steps done."""
+        _check(lib.mtr_checkpoint_save(self._h, _os.fsencode(path), steps_done), "checkpoint_save")
+
+    def checkpoint_load(self, path: str) -> int:
+        """Replaces the MACE state by a checkpoint of the same inputs; returns its count of steps done."""
+        n = _ct.c_int()
+        _check(lib.mtr_checkpoint_load(self._h, _os.fsencode(path), _ct.byref(n)), "checkpoint_load")
+        return n.value
 
     def get_volume(self, which: int, agent: int = 0) -> _np.ndarray:
         o = _np.empty(self.shape, _np.float32)

@@ -66,6 +66,8 @@ _PROTOS = {
     "mtr_mace_init": (_i, [_v]),
     "mtr_mace_solve": (_i, [_v, _i, _i, _f, _f, _i, _pf]),
     "mtr_mace_continue": (_i, [_v, _i, _i, _f, _f, _d, _pi, _pf]),
+    "mtr_checkpoint_save": (_i, [_v, ct.c_char_p, _i]),
+    "mtr_checkpoint_load": (_i, [_v, ct.c_char_p, _pi]),
     "mtr_get_volume": (_i, [_v, _i, _i, _pf]),
     "mtr_set_volume": (_i, [_v, _i, _i, _pf]),
     "mtr_wbar_device": (_i, [_v, ct.POINTER(_v)]),

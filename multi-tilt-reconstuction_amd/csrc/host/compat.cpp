@@ -83,7 +83,10 @@ void helpInfo()
         "  --precision P   fp32 | bf16 | fp16 storage (default fp32)\n"
         "  --form F        fast | plain (default fast)\n"
         "  --device D      GPU index (default 0)\n"
-        "  --tune-cache F  keep the autotuner's tile choices in file F across runs\n");
+        "  --tune-cache F  keep the autotuner's tile choices in file F across runs\n"
+        "  --tol T         stop early once the consensus difference changes by <= T (relative)\n"
+        "  --checkpoint F  write the MACE state to F after the solve\n"
+        "  --resume F      continue from checkpoint F (same inputs) instead of the initial model\n");
 }
 
 int readOptions(int argc, char **argv, Options *opts)
@@ -100,6 +103,9 @@ int readOptions(int argc, char **argv, Options *opts)
         {"form", required_argument, nullptr, 1003},
         {"device", required_argument, nullptr, 1004},
         {"tune-cache", required_argument, nullptr, 1005},
+        {"tol", required_argument, nullptr, 1006},
+        {"checkpoint", required_argument, nullptr, 1007},
+        {"resume", required_argument, nullptr, 1008},
         {nullptr, 0, nullptr, 0},
     };
     optind = 1;
@@ -130,6 +136,9 @@ int readOptions(int argc, char **argv, Options *opts)
             case 1004: opts->device = std::atoi(optarg); break;
             // the library reads and extends the cache named by MTR_TUNE_CACHE
             case 1005: setenv("MTR_TUNE_CACHE", optarg, 1); break;
+            case 1006: opts->tol = std::atof(optarg); break;
+            case 1007: opts->checkpoint = optarg; break;
+            case 1008: opts->resume = optarg; break;
             case 'i': {
                 // K,mrc_1..mrc_K,txbr_1..txbr_K (ReadAndInitialOptions.cpp:104-125)
                 std::stringstream ss(optarg);
@@ -156,6 +165,14 @@ int readOptions(int argc, char **argv, Options *opts)
         }
     }
     if (opts->sirt_iters < 0 || opts->iter_times < 0) return -1;
+    if (!(opts->tol >= 0.0)) {
+        std::fprintf(stderr, "--tol must be >= 0\n");
+        return -1;
+    }
+    if (opts->form == MTR_FORM_PLAIN && (opts->tol > 0.0 || !opts->checkpoint.empty() || !opts->resume.empty())) {
+        std::fprintf(stderr, "--tol, --checkpoint and --resume run with --form fast only\n");
+        return -1;
+    }
     return 1;
 }
 
@@ -313,8 +330,21 @@ int reconstructModel(std::vector<InputData> &inputs, Options *opts, std::vector<
     const VolumeSize &v = inputs[0].vol;
     w_avg.assign((size_t)v.X * v.Y * v.Z, 0.0f);
     const auto t0 = std::chrono::steady_clock::now();
-    rc = mtr_mace_solve(c.h, opts->iter_times, opts->sirt_iters, opts->sirt_step, opts->mace_step, opts->form,
-                        w_avg.data());
+    if (opts->tol <= 0.0 && opts->checkpoint.empty() && opts->resume.empty()) {
+        rc = mtr_mace_solve(c.h, opts->iter_times, opts->sirt_iters, opts->sirt_step, opts->mace_step, opts->form,
+                            w_avg.data());
+        opts->steps_done = rc == MTR_OK ? opts->iter_times : 0;
+    } else {
+        // convergence control and checkpoints run the fast form only
+        int before = 0, run = 0;
+        rc = opts->resume.empty() ? mtr_mace_init(c.h) : mtr_checkpoint_load(c.h, opts->resume.c_str(), &before);
+        if (rc == MTR_OK)
+            rc = mtr_mace_continue(c.h, opts->iter_times, opts->sirt_iters, opts->sirt_step, opts->mace_step,
+                                   opts->tol, &run, w_avg.data());
+        opts->steps_done = before + run;
+        if (rc == MTR_OK && !opts->checkpoint.empty())
+            rc = mtr_checkpoint_save(c.h, opts->checkpoint.c_str(), opts->steps_done);
+    }
     const auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     return rc;

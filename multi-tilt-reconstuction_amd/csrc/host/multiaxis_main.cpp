@@ -46,7 +46,8 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "reconstruction failed (%d): %s\n", sr, mtr_last_error());
         return 1;
     }
-    std::printf("Total time: %.6f s (wall)\n", secs);
+    std::printf("Total time: %.6f s (wall), MACE steps done: %d\n", secs, opts.steps_done);
+    if (!opts.checkpoint.empty()) std::printf("Checkpoint: %s\n", opts.checkpoint.c_str());
     OutputModel res;
     res.slc = divideSlice(0, 1, v.Z, v.Zstart);
     res.initialize(v);

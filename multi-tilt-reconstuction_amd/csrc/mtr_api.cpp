@@ -1665,6 +1665,112 @@ int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host)
     return MTR_OK;
 }
 
+// ---- MACE checkpoint file ------------------------------------------
+// A 64-byte header, then the K + 1 state volumes w_0 .. w_{K-1}, w-bar as
+// float32 [Z][Y][X], little-endian.  The geometry hash ties a checkpoint
+// to the inputs it was taken from (every agent's detector size and
+// projection coefficients, in agent order).
+namespace {
+
+struct CkptHeader {
+    char magic[8];
+    int32_t K, X, Y, Z, Xstart, Ystart, Zstart;
+    int32_t steps_done;
+    int32_t precision;
+    int32_t reserved0;
+    uint64_t geometry;
+    uint64_t reserved1;
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+constexpr char kCkptMagic[8] = {'M', 'T', 'R', 'C', 'K', 'P', 'T', '1'};
+
+uint64_t geometry_hash(const mtr_ctx *ctx)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (const Agent &ag : ctx->agents) {
+        const int32_t d[3] = {ag.prj.X, ag.prj.Y, ag.prj.AngN};
+        h = fnv(d, sizeof d, h);
+        h = fnv(ag.xc.data(), sizeof(double) * ag.xc.size(), h);
+        h = fnv(ag.yc.data(), sizeof(double) * ag.yc.size(), h);
+    }
+    return h;
+}
+
+struct File {
+    FILE *f = nullptr;
+    ~File()
+    {
+        if (f) std::fclose(f);
+    }
+};
+
+}  // namespace
+
+int mtr_checkpoint_save(mtr_ctx *ctx, const char *path, int steps_done)
+{
+    if (!ctx || !path) return fail(MTR_ERR_ARG, "null argument");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    if (steps_done < 0) return fail(MTR_ERR_ARG, "negative step count");
+    MTR_TRY(set_device(ctx));
+    CkptHeader h{};
+    std::memcpy(h.magic, kCkptMagic, sizeof h.magic);
+    h.K = (int32_t)ctx->agents.size();
+    h.X = ctx->vol.X, h.Y = ctx->vol.Y, h.Z = ctx->vol.Z;
+    h.Xstart = ctx->vol.Xstart, h.Ystart = ctx->vol.Ystart, h.Zstart = ctx->vol.Zstart;
+    h.steps_done = steps_done;
+    h.precision = ctx->precision;
+    h.geometry = geometry_hash(ctx);
+    File out;
+    if (!(out.f = std::fopen(path, "wb"))) return fail(MTR_ERR_ARG, "cannot create %s", path);
+    if (std::fwrite(&h, sizeof h, 1, out.f) != 1) return fail(MTR_ERR_ARG, "write failed: %s", path);
+    std::vector<float> buf((size_t)ctx->N);
+    for (int v = 0; v <= h.K; v++) {
+        MTR_TRY(copy_volume_out(ctx, v < h.K ? ctx->agents[v].d_w : ctx->d_wbar, buf.data()));
+        if (std::fwrite(buf.data(), sizeof(float), buf.size(), out.f) != buf.size())
+            return fail(MTR_ERR_ARG, "write failed: %s", path);
+    }
+    const int rc = std::fclose(out.f);
+    out.f = nullptr;
+    return rc == 0 ? MTR_OK : fail(MTR_ERR_ARG, "write failed: %s", path);
+}
+
+int mtr_checkpoint_load(mtr_ctx *ctx, const char *path, int *steps_done)
+{
+    if (!ctx || !path) return fail(MTR_ERR_ARG, "null argument");
+    if (ctx->agents.empty()) return fail(MTR_ERR_STATE, "no agents");
+    MTR_TRY(set_device(ctx));
+    File in;
+    if (!(in.f = std::fopen(path, "rb"))) return fail(MTR_ERR_ARG, "cannot open %s", path);
+    CkptHeader h{};
+    if (std::fread(&h, sizeof h, 1, in.f) != 1 || std::memcmp(h.magic, kCkptMagic, sizeof h.magic) != 0)
+        return fail(MTR_ERR_SHAPE, "%s is not a MACE checkpoint", path);
+    if (h.K != (int32_t)ctx->agents.size() || h.X != ctx->vol.X || h.Y != ctx->vol.Y || h.Z != ctx->vol.Z ||
+        h.Xstart != ctx->vol.Xstart || h.Ystart != ctx->vol.Ystart || h.Zstart != ctx->vol.Zstart)
+        return fail(MTR_ERR_SHAPE, "checkpoint %s holds %d agents on a %dx%dx%d box, the context %d on %dx%dx%d",
+                    path, h.K, h.X, h.Y, h.Z, (int)ctx->agents.size(), ctx->vol.X, ctx->vol.Y, ctx->vol.Z);
+    if (h.geometry != geometry_hash(ctx))
+        return fail(MTR_ERR_SHAPE, "checkpoint %s was taken with other projection geometry", path);
+    if (h.steps_done < 0) return fail(MTR_ERR_SHAPE, "checkpoint %s: bad step count", path);
+    // the whole payload must be there before any state is replaced
+    const long long need = (long long)sizeof h + (long long)(h.K + 1) * ctx->N * (long long)sizeof(float);
+    if (std::fseek(in.f, 0, SEEK_END) != 0 || std::ftell(in.f) != need)
+        return fail(MTR_ERR_SHAPE, "checkpoint %s is truncated or too long", path);
+    // pass 1 checks every value, pass 2 uploads
+    std::vector<float> buf((size_t)ctx->N);
+    for (int pass = 0; pass < 2; pass++) {
+        if (std::fseek(in.f, sizeof h, SEEK_SET) != 0) return fail(MTR_ERR_ARG, "read failed: %s", path);
+        for (int v = 0; v <= h.K; v++) {
+            if (std::fread(buf.data(), sizeof(float), buf.size(), in.f) != buf.size())
+                return fail(MTR_ERR_ARG, "read failed: %s", path);
+            if (pass == 0 && first_nonfinite(buf.data(), ctx->N) >= 0)
+                return fail(MTR_ERR_NONFINITE, "checkpoint %s: NaN or Inf in volume %d", path, v);
+            if (pass == 1) MTR_TRY(copy_volume_in(ctx, v < h.K ? ctx->agents[v].d_w : ctx->d_wbar, buf.data()));
+        }
+    }
+    if (steps_done) *steps_done = h.steps_done;
+    return MTR_OK;
+}
+
 int mtr_wbar_device(mtr_ctx *ctx, void **dev_ptr)
 {
     if (!ctx || !dev_ptr) return fail(MTR_ERR_ARG, "null argument");

@@ -6,10 +6,13 @@ adds the two controls a long solve needs, on the same step kernels:
 
 * continuing from the state (w_a, w-bar) is bit-identical to a solve that
   never stopped, in the same context or in a new one restored from a
-  checkpoint taken with get_volume, in every storage precision;
+  checkpoint (get_volume/set_volume_data, or a checkpoint file), in every
+  storage precision;
 * with rel_tol > 0 the loop stops after the first step whose FP64 consensus
   difference sum_a ||w_a - w-bar||^2 changed by at most rel_tol of the
-  previous step's value, and the result is that of a solve of that many steps.
+  previous step's value, and the result is that of a solve of that many steps;
+* a checkpoint file of other inputs, or a damaged one, is refused before
+  any state is replaced.
 """
 import sys
 
@@ -135,3 +138,67 @@ def test_continue_errors(mtr):
         # without init the state starts at zero: a solve from the zero volume
         out, n = c.mace_continue(1, 1, 0.5, 0.5)
         assert n == 1 and np.isfinite(out).all()
+
+
+def test_checkpoint_file_round_trip(mtr, tmp_path):
+    pb = problem(mtr, seed=36)
+    path = str(tmp_path / "mace.ckpt")
+    say("checkpoint file")
+    with load(mtr, pb) as c:
+        whole = c.mace_solve(3, 2, 0.5, 0.5)
+    with load(mtr, pb) as c:
+        c.mace_solve(2, 2, 0.5, 0.5, out=False)
+        c.checkpoint_save(path, 2)
+    n = pb.vol.X * pb.vol.Y * pb.vol.Z
+    raw = np.fromfile(path, np.uint8)
+    assert raw.size == 64 + 4 * n * (pb.K + 1) and raw[:8].tobytes() == b"MTRCKPT1"
+    with load(mtr, pb) as c:
+        assert c.checkpoint_load(path) == 2
+        resumed, steps = c.mace_continue(1, 2, 0.5, 0.5)
+    assert steps == 1 and np.array_equal(whole, resumed)
+
+
+def test_bad_checkpoints_leave_the_state(mtr, tmp_path):
+    pb = problem(mtr, seed=37)
+    good = tmp_path / "good.ckpt"
+    with load(mtr, pb) as c:
+        c.mace_solve(1, 1, 0.5, 0.5, out=False)
+        c.checkpoint_save(str(good), 1)
+    raw = good.read_bytes()
+    cases = {
+        "magic": (b"XXXXXXXX" + raw[8:], -2),
+        "short": (raw[:-4], -2),
+        "long": (raw + b"\0\0\0\0", -2),
+        "nan": (raw[:100] + np.float32(np.nan).tobytes() + raw[104:], -5),
+    }
+    say("bad checkpoints")
+    with load(mtr, pb) as c:
+        c.mace_init()
+        before = [c.get_volume(mtr.VOL_W, a) for a in range(pb.K)] + [c.get_volume(mtr.VOL_WBAR)]
+        for name, (data, code) in cases.items():
+            p = tmp_path / f"{name}.ckpt"
+            p.write_bytes(data)
+            with pytest.raises(Exception) as ei:
+                c.checkpoint_load(str(p))
+            assert ei.value.code == code, name
+        after = [c.get_volume(mtr.VOL_W, a) for a in range(pb.K)] + [c.get_volume(mtr.VOL_WBAR)]
+        assert all(np.array_equal(x, y) for x, y in zip(before, after))
+        with pytest.raises(Exception) as ei:
+            c.checkpoint_load(str(tmp_path / "missing.ckpt"))
+        assert ei.value.code == ERR_ARG
+    # other inputs: one agent fewer, or other coefficients
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        c.add_agent(pb.prjs[0], pb.xcs[0], pb.ycs[0], pb.ps[0])
+        with pytest.raises(Exception) as ei:
+            c.checkpoint_load(str(good))
+        assert ei.value.code == -2
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        for a in range(pb.K):
+            xc = pb.xcs[a].copy()
+            xc[0] += 0.25 if a == 1 else 0.0
+            c.add_agent(pb.prjs[a], xc, pb.ycs[a], pb.ps[a])
+        with pytest.raises(Exception) as ei:
+            c.checkpoint_load(str(good))
+        assert ei.value.code == -2

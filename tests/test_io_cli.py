@@ -123,6 +123,12 @@ def test_cli_help_and_bad_input():
     assert r.returncode == 2
     r = subprocess.run([CLI, "-q"], capture_output=True, text=True)
     assert r.returncode == 2
+    # convergence control and checkpoints need the fast form; tolerances are >= 0
+    r = subprocess.run([CLI, "-i", "1,a.st,a.txbr", "-o", "x.mrc", "--form", "plain", "--tol", "0.1"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2 and "--form fast" in r.stderr
+    r = subprocess.run([CLI, "-i", "1,a.st,a.txbr", "-o", "x.mrc", "--tol", "-1"], capture_output=True, text=True)
+    assert r.returncode == 2
 
 
 def write_problem(mtr, pb, d):
@@ -175,3 +181,42 @@ def test_reference_shaped_host_functions(mtr, orc, tmp_path):
     assert np.array_equal(got["z"], z)
     assert np.array_equal(got["mann"], orc.mann(got["sirt"], got["bpt"], z, 0.5))
     assert np.array_equal(got["mann_cpu_name"], got["mann"])
+
+
+@pytest.mark.gpu
+def test_cli_checkpoint_resume_and_early_stop(mtr, tmp_path):
+    pb = mtr.synth.make_problem("ckpt", 30, 26, 10, 6, [0.0, 45.0], seed=24, amin=2, amax=6,
+                                theta=np.linspace(-60, 60, 9))
+    sts, txs = write_problem(mtr, pb, tmp_path)
+    inp = ",".join([str(pb.K)] + sts + txs)
+    ck = str(tmp_path / "state.ckpt")
+
+    def cli(out, n, *extra):
+        print("[gpu] cli", n, *extra, flush=True)
+        r = subprocess.run([CLI, "-i", inp, "-o", str(tmp_path / out), "-n", str(n), "-s", "0.5", "-m", "0.5",
+                            "--sirt-iters", "2", *extra], capture_output=True, text=True, timeout=120)
+        return r
+
+    r = cli("whole.mrc", 4)
+    assert r.returncode == 0, r.stderr
+    assert "MACE steps done: 4" in r.stdout
+    r = cli("half.mrc", 2, "--checkpoint", ck)
+    assert r.returncode == 0 and "MACE steps done: 2" in r.stdout, r.stderr
+    r = cli("resumed.mrc", 2, "--resume", ck)
+    assert r.returncode == 0 and "MACE steps done: 4" in r.stdout, r.stderr
+    # two runs of two steps through a checkpoint file are one run of four
+    assert np.array_equal(mtr.mrc.read(str(tmp_path / "whole.mrc")), mtr.mrc.read(str(tmp_path / "resumed.mrc")))
+    # a generous tolerance stops well before the step limit
+    r = cli("early.mrc", 40, "--tol", "0.5")
+    assert r.returncode == 0, r.stderr
+    done = int(r.stdout.split("MACE steps done:")[1].split()[0])
+    assert 2 <= done < 40
+    # a checkpoint of other inputs is refused
+    other = mtr.synth.make_problem("ckpt", 30, 26, 10, 6, [0.0, 60.0], seed=24, amin=2, amax=6,
+                                   theta=np.linspace(-60, 60, 9))
+    d2 = tmp_path / "other"
+    d2.mkdir()
+    sts2, txs2 = write_problem(mtr, other, d2)
+    r = subprocess.run([CLI, "-i", ",".join([str(other.K)] + sts2 + txs2), "-o", str(d2 / "o.mrc"), "-n", "1",
+                        "--resume", ck], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "geometry" in r.stderr
