@@ -324,6 +324,7 @@ struct Agent {
     bool lanes_k = false;  // forward projection with lanes along detector rows
     BpMode bpm;            // back-projection kernel choice
     int fp_nc = 0;         // > 0: streaming forward projection with this candidate count
+    int fp_g = 1;          // streaming forward projection: detector columns per wave (1, 2, 4)
     int npartial = 0;      // per-block partials written by the last FP launch
     int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
     int64_t st[3] = {0, 0, 0};
@@ -550,7 +551,10 @@ struct Batch {
     AgentDev *d_tab = nullptr;
 };
 
-int fp_kind_key(const Agent &a) { return a.col.on ? 100 + a.col.rw : (a.fp_nc ? 10 + a.fp_nc : (a.lanes_k ? 1 : 0)); }
+int fp_kind_key(const Agent &a)
+{
+    return a.col.on ? 100 + a.col.rw : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g : (a.lanes_k ? 1 : 0));
+}
 int bp_kind_key(const Agent &a)
 {
     return a.col.on ? 100 + a.col.rw : (a.bpm.sep ? 10 + 4 * a.bpm.W + (a.bpm.rw == 32 ? 1 : (a.bpm.rw == 16 ? 2 : 3)) : 0);
@@ -575,7 +579,7 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
                 mr = std::max(mr, ag.npartial);
                 nu = std::max(nu, ag.prj.X);
             } else {
-                ag.npartial = ag.fp_nc ? fp_stream_blocks(d) : ag.nfp;
+                ag.npartial = ag.fp_nc ? fp_stream_blocks(d, ag.fp_g) : ag.nfp;
                 mb = std::max(mb, ag.npartial);
             }
         }
@@ -584,7 +588,7 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
             HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
             HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
         } else if (a0.fp_nc) {
-            HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, eng->stream));
+            HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, eng->stream));
         } else {
             HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, eng->stream));
         }
@@ -938,6 +942,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.fp_nc = fp_stream_nc(plan, xc, &sW);
     // the streaming FP addresses the volume with 32-bit byte offsets
     if ((uint64_t)ctx->vol.X * ctx->vol.Y * ctx->vol.Z * elem_size(ctx->precision) > 0xFFFFFFFFull) ag.fp_nc = 0;
+    ag.fp_g = 2;
+    if (const char *g = std::getenv("MTR_FP_COLS")) ag.fp_g = std::atoi(g) == 4 ? 4 : (std::atoi(g) == 1 ? 1 : 2);
     ag.fast = ag.fp_nc ? sW : 0;
     if (!ag.fp_nc && !ag.bpm.sep) {
         ag.col = col_mode(xc, yc, A, *prj, 16);
@@ -1009,7 +1015,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
             return undo(fail(MTR_ERR_HIP, "column table upload failed"));
     }
     ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
-    ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag))),
+    ALLOC(ag.d_partial, sizeof(double) * std::max(std::max(std::max(ag.nfp, fp_stream_blocks(make_dev(ctx, ag), 1)),
                                                            fp_col_res_blocks(make_dev(ctx, ag))),
                                                   1));
     ALLOC(ag.d_scalar, sizeof(double));

@@ -10,8 +10,9 @@ int bp_blocks(const AgentDev &ag, int tz);
 int consensus_blocks(int64_t N);
 // Row-separable forward projection streaming one ray per wave over 64
 // planes (plane axis contiguous, u independent of it), fused residual.
-int fp_stream_blocks(const AgentDev &ag);
-hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st);
+int fp_stream_blocks(const AgentDev &ag, int cols_per_wave);
+hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, int cols_per_wave,
+                            hipStream_t st);
 // Row-separable back-projection (runs along the plane axis W).
 // rw: voxels per thread along W (8, 16 or 32).
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw);

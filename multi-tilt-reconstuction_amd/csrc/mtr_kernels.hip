@@ -394,20 +394,28 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
 // ------------------------------------------------------------------------
 // Forward projection of row-separable agents whose plane axis W is the
 // fastest-varying axis of the volume and whose u does not depend on W.
-// Block: 4 waves, each one detector column i of one view; the 64 lanes are
-// 64 consecutive detector rows k, i.e. 64 adjacent planes W(k).  All lanes
-// share the ray path, so the support is generated once per wave: lane q
-// builds candidate q of the current chunk, a ballot keeps the taps with
-// nonzero weight, and their (offset, weight) pairs are broadcast with
-// readlane; each tap is then one coalesced 256-byte load and one FMA for
-// 64 pixels.  Taps are consumed in (line, candidate) order, the order of
-// the per-lane path, so both give identical sums.
-template <class T, int NC>
+// Block: 4 waves, each G adjacent detector columns i0 .. i0 + G - 1 of one
+// view; the 64 lanes are 64 consecutive detector rows k, i.e. 64 adjacent
+// planes W(k).  All lanes share the ray paths, so the support is generated
+// once per wave: lane q builds candidate voxel q of the current chunk (the
+// union over the G columns of each line's lattice points), a ballot keeps
+// the voxels with a nonzero weight for any of the columns, and their byte
+// offsets and G weights go to the wave's LDS list; each listed voxel is
+// then one coalesced 256-byte load and G FMAs for 64 x G pixels.  A voxel
+// row that the per-column scheme loads once per column it touches (about
+// twice) is loaded once per wave, which is what bounds this kernel: the
+// vector-memory data path.  Every column still sums its own taps in (line,
+// candidate) order, the order of the per-lane path, and the weights are the
+// same expressions, so the sums are identical for every G (the extra FMAs
+// add exact zeros).
+template <class T, int NC, int G>
 __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ ags)
 {
-    __shared__ int2 taps[4][72];  // per wave: (offset, weight bits), 64 + 8 padding
+    constexpr int TP = 72;  // per wave: 64 listed voxels + 8 padding
+    __shared__ int tap_off[4][TP];
+    __shared__ float tap_w[4][G][TP];
     const AgentDev &ag = ags[blockIdx.y];
-    const int nti = (ag.nu + 3) / 4, ntk = (ag.nv + 63) / 64;
+    const int nti = (ag.nu + 4 * G - 1) / (4 * G), ntk = (ag.nv + 63) / 64;
     // XCD-aware order: each XCD takes a contiguous range of (plane group,
     // column group, view) with the view fastest, so the rays in flight on
     // one XCD are all views of a few adjacent columns of one plane group and
@@ -425,96 +433,138 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const int bi = b % nti;
     const int bk = b / nti;
     const int lane = threadIdx.x & 63, wid = wave_id();
-    const int i = bi * 4 + wid;
+    const int i0 = (bi * 4 + wid) * G;
+    const int gn = min(G, ag.nu - i0);  // this wave's columns on the detector (may be <= 0)
     const int k = bk * 64 + lane;
     const FpPlan pl = ag.plan[a];
     const ViewCoef vc = ag.coef[a];
     const T *x = reinterpret_cast<const T *>(ag.x);
     const int Wlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.W), Whi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.W) - 1;
     const int w = k * pl.wsign + (int)pl.woff;
-    const bool lane_ok = i < ag.nu && k < ag.nv && w >= Wlo && w <= Whi;
+    const bool row_ok = k < ag.nv && w >= Wlo && w <= Whi;
     const int64_t sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L), sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M),
                   sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W);
     const int64_t base0 = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
     // per-lane 32-bit byte offset of the lane's plane (the volume is below
     // 4 GiB: the host selects this kernel only then)
-    const uint32_t lane_off = (uint32_t)(((lane_ok ? w : Wlo) - Wlo) * sW) * (uint32_t)sizeof(T);
-    float acc = 0.f;
-    if (i < ag.nu) {
+    const uint32_t lane_off = (uint32_t)(((row_ok ? w : Wlo) - Wlo) * sW) * (uint32_t)sizeof(T);
+    float acc[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) acc[g] = 0.f;
+    if (gn > 0) {
         const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
         const int Mlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.M), Mhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.M) - 1;
-        const double ms = pl.m0 + i * pl.mi;  // aW == 0: no plane term
-        double llo = Llo, lhi = Lhi;
-        bool any = true;
-        if (pl.ml != 0.0) {
-            double t0 = (Mlo - pl.Em - 1.0 - ms) / pl.ml, t1 = (Mhi + pl.Em + 1.0 - ms) / pl.ml;
-            llo = fmax(llo, fmin(t0, t1));
-            lhi = fmin(lhi, fmax(t0, t1));
-        } else if (ms < Mlo - pl.Em - 1.0 || ms > Mhi + pl.Em + 1.0) {
-            any = false;
+        double ms[G];  // aW == 0: no plane term
+        // lines crossing the box, united over the wave's columns
+        int l0 = Lhi + 1, l1 = Llo - 1;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            ms[g] = pl.m0 + (i0 + g) * pl.mi;
+            if (g >= gn) continue;
+            double llo = Llo, lhi = Lhi;
+            bool any = true;
+            if (pl.ml != 0.0) {
+                double t0 = (Mlo - pl.Em - 1.0 - ms[g]) / pl.ml, t1 = (Mhi + pl.Em + 1.0 - ms[g]) / pl.ml;
+                llo = fmax(llo, fmin(t0, t1));
+                lhi = fmin(lhi, fmax(t0, t1));
+            } else if (ms[g] < Mlo - pl.Em - 1.0 || ms[g] > Mhi + pl.Em + 1.0) {
+                any = false;
+            }
+            if (any && llo <= lhi) {
+                l0 = min(l0, max(Llo, (int)floor(llo)));
+                l1 = max(l1, min(Lhi, (int)ceil(lhi)));
+            }
         }
-        if (any && llo <= lhi) {
-            const int l0 = max(Llo, (int)floor(llo)), l1 = min(Lhi, (int)ceil(lhi));
+        if (l0 <= l1) {
             const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
             const unsigned mspan = (unsigned)(Mhi - Mlo);
-            const int n = (l1 - l0 + 1) * NC;
+            // candidate slots per line: the columns' lattice points start
+            // (G - 1) |mi| apart at most (one more slot where that spread is
+            // an integer, against rounding of the starts)
+            int ncu = NC;
+            if (G > 1) {
+                const double spread = (G - 1) * fabs(pl.mi);
+                ncu += (int)ceil(spread) + (fabs(spread - rint(spread)) < 1e-6 ? 1 : 0);
+            }
+            const int n = (l1 - l0 + 1) * ncu;
             // scalar base: voxel (Llo, Mlo) of the box's first plane
             const auto xc = uniform_base(x + base0 + (int64_t)Wlo * sW + (int64_t)Llo * sL + (int64_t)Mlo * sM);
             for (int c0 = 0; c0 < n; c0 += 64) {
-                // candidate c of the ray: line l0 + c / NC, point c % NC
+                // candidate c: line l0 + c / ncu, voxel mb + c % ncu
                 const int c = c0 + lane;
-                const int l = l0 + c / NC, j = c - (c / NC) * NC;
-                const double mc = ms + l * pl.ml;
-                const double mf = floor(mc - pl.Em) + 1.0;
-                const int m = (int)mf + j;
-                // same weight as the per-lane path
+                const int l = l0 + c / ncu, j = c - (c / ncu) * ncu;
+                double mc[G], mf[G];
+                double mb = 0.0;
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+                    mc[g] = ms[g] + l * pl.ml;
+                    mf[g] = floor(mc[g] - pl.Em) + 1.0;
+                    mb = g == 0 ? mf[0] : fmin(mb, mf[g]);
+                }
+                const int m = (int)mb + j;
                 const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
-                const float wt = ok ? tent(fma((double)j, aM, aM * (mf - mc))) : 0.0f;
+                // weight of column g: point m - mf[g] of its line, the same
+                // expression as the per-lane path
+                float wt[G];
+                bool anyw = false;
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+                    const int jg = m - (int)mf[g];
+                    wt[g] = ok && g < gn && jg >= 0 && jg < NC ? tent(fma((double)jg, aM, aM * (mf[g] - mc[g])))
+                                                                 : 0.0f;
+                    anyw = anyw || wt[g] > 0.0f;
+                }
                 // byte offset from voxel (Llo, Mlo) of the first plane, >= 0
                 const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) * (int)sizeof(T) : 0;
-                // compact the nonzero taps of this chunk into the wave's LDS
+                // compact the listed voxels of this chunk into the wave's LDS
                 // list (order preserved), padded to a multiple of 8
-                const uint64_t mask = __ballot(wt > 0.0f);
+                const uint64_t mask = __ballot(anyw);
                 const int cnt = __popcll(mask);
                 const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                if (wt > 0.0f) taps[wid][pos] = make_int2(off, __float_as_int(wt));
-                if (lane >= cnt && lane < cnt + 8) taps[wid][lane] = make_int2(0, 0);
-                if (lane < 8 && 64 + lane < cnt + 8) taps[wid][64 + lane] = make_int2(0, 0);
+                if (anyw) {
+                    tap_off[wid][pos] = off;
+#pragma unroll
+                    for (int g = 0; g < G; g++) tap_w[wid][g][pos] = wt[g];
+                }
+                const int pad = lane >= cnt && lane < cnt + 8 ? lane : (lane < 8 && 64 + lane < cnt + 8 ? 64 + lane : -1);
+                if (pad >= 0) {
+                    tap_off[wid][pad] = 0;
+#pragma unroll
+                    for (int g = 0; g < G; g++) tap_w[wid][g][pad] = 0.0f;
+                }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // taps in groups of 8 held in two register sets: the next
+                // voxels in groups of 8 held in two register sets: the next
                 // group's loads are in flight while the current one is summed
                 const int cnt8 = (cnt + 7) & ~7;
-                float xa[8], wa[8], xb[8], wb[8];
-                auto fetch = [&](int e, float(&xv)[8], float(&ww)[8]) {
+                float xa[8], xb[8];
+                auto fetch = [&](int e, float(&xv)[8]) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const int2 tp = taps[wid][e + q];
-                        ww[q] = __int_as_float(tp.y);
-                        xv[q] = ldb<T>(xc, (uint32_t)tp.x + lane_off);
-                    }
+                    for (int q = 0; q < 8; q++) xv[q] = ldb<T>(xc, (uint32_t)tap_off[wid][e + q] + lane_off);
                 };
-                auto sum = [&](const float(&xv)[8], const float(&ww)[8]) {
+                auto sum = [&](int e, const float(&xv)[8]) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) acc = fmaf(ww[q], xv[q], acc);
+                    for (int q = 0; q < 8; q++)
+#pragma unroll
+                        for (int g = 0; g < G; g++) acc[g] = fmaf(tap_w[wid][g][e + q], xv[q], acc[g]);
                 };
                 if (cnt > 0) {
-                    fetch(0, xa, wa);
+                    fetch(0, xa);
                     int e = 0;
                     for (; e + 16 < cnt8; e += 16) {
-                        fetch(e + 8, xb, wb);
-                        sum(xa, wa);
-                        fetch(e + 16, xa, wa);
-                        sum(xb, wb);
+                        fetch(e + 8, xb);
+                        sum(e, xa);
+                        fetch(e + 16, xa);
+                        sum(e + 8, xb);
                     }
                     if (e + 8 < cnt8) {
-                        fetch(e + 8, xb, wb);
-                        sum(xa, wa);
-                        sum(xb, wb);
+                        fetch(e + 8, xb);
+                        sum(e, xa);
+                        sum(e + 8, xb);
                     } else {
-                        sum(xa, wa);
+                        sum(e, xa);
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -524,17 +574,21 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
         }
     }
     double r2 = 0.0;
-    if (lane_ok) {
-        // with the row-fastest layout (pk = 1) the 64 lanes touch 64
-        // consecutive elements of p, c and r
-        const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)i * ag.pi;
-        const float cc = ag.c[nidx];
-        float sv = acc;
-        if (cc != 0.0f) sv = sv / cc;
-        const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), nidx) - sv;
-        stf<T>(reinterpret_cast<T *>(ag.r), nidx, rv);
-        const double rs = (double)rnd<T>(rv);  // the stored value
-        r2 = rs * rs;
+    if (row_ok) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            if (g >= gn) break;
+            // with the row-fastest layout (pk = 1) the 64 lanes touch 64
+            // consecutive elements of p, c and r
+            const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)(i0 + g) * ag.pi;
+            const float cc = ag.c[nidx];
+            float sv = acc[g];
+            if (cc != 0.0f) sv = sv / cc;
+            const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), nidx) - sv;
+            stf<T>(reinterpret_cast<T *>(ag.r), nidx, rv);
+            const double rs = (double)rnd<T>(rv);  // the stored value
+            r2 += rs * rs;
+        }
     }
     __shared__ double sh[4];
     const double tsum = block_sum<256>(r2, sh);
@@ -1977,23 +2031,34 @@ hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, i
     return launch_bp_t<float>(d_ags, nag, max_blocks, tz, bpt, lambda, src, lds_cap, st);
 }
 
-int fp_stream_blocks(const AgentDev &ag) { return (((ag.nu + 3) / 4) * ((ag.nv + 63) / 64) * ag.A + 7) / 8 * 8; }
+int fp_stream_blocks(const AgentDev &ag, int g)
+{
+    return (((ag.nu + 4 * g - 1) / (4 * g)) * ((ag.nv + 63) / 64) * ag.A + 7) / 8 * 8;
+}
 
-template <class T>
-static hipError_t launch_fp_stream_t(const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
+template <class T, int G>
+static hipError_t launch_fp_stream_g(const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
 {
     dim3 grid(max_blocks, nag), block(256);
-    if (nc == 2) hipLaunchKernelGGL((k_fp_stream<T, 2>), grid, block, 0, st, d_ags);
-    else if (nc == 3) hipLaunchKernelGGL((k_fp_stream<T, 3>), grid, block, 0, st, d_ags);
-    else hipLaunchKernelGGL((k_fp_stream<T, 4>), grid, block, 0, st, d_ags);
+    if (nc == 2) hipLaunchKernelGGL((k_fp_stream<T, 2, G>), grid, block, 0, st, d_ags);
+    else if (nc == 3) hipLaunchKernelGGL((k_fp_stream<T, 3, G>), grid, block, 0, st, d_ags);
+    else hipLaunchKernelGGL((k_fp_stream<T, 4, G>), grid, block, 0, st, d_ags);
     return hipGetLastError();
 }
 
-hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, hipStream_t st)
+template <class T>
+static hipError_t launch_fp_stream_t(const AgentDev *d_ags, int nag, int max_blocks, int nc, int g, hipStream_t st)
 {
-    if (prec == MTR_PREC_BF16) return launch_fp_stream_t<__hip_bfloat16>(d_ags, nag, max_blocks, nc, st);
-    if (prec == MTR_PREC_FP16) return launch_fp_stream_t<__half>(d_ags, nag, max_blocks, nc, st);
-    return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, st);
+    if (g == 4) return launch_fp_stream_g<T, 4>(d_ags, nag, max_blocks, nc, st);
+    if (g == 2) return launch_fp_stream_g<T, 2>(d_ags, nag, max_blocks, nc, st);
+    return launch_fp_stream_g<T, 1>(d_ags, nag, max_blocks, nc, st);
+}
+
+hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, int g, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_fp_stream_t<__hip_bfloat16>(d_ags, nag, max_blocks, nc, g, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_stream_t<__half>(d_ags, nag, max_blocks, nc, g, st);
+    return launch_fp_stream_t<float>(d_ags, nag, max_blocks, nc, g, st);
 }
 
 // odd (in float2 units) so that a column of 32 rows written by 32 lanes

@@ -385,3 +385,42 @@ def test_autotuned_tiles_do_not_change_results(mtr):
         for q in range(pb.K):
             run, lanes = tuned.agent_tiles(q)
             assert run in (0, 8, 16, 32) and lanes in (0, 1, 2)  # 0: the general kernel
+
+
+STREAM_GEOMS = [
+    ("slab0", dict(phis=[0.0])),
+    ("slab90", dict(phis=[90.0])),
+    ("ragged", dict(phis=[0.0], X=37, Y=23, Z=11, nu=29, nv=41)),
+    ("origin", dict(phis=[90.0], origin=(5, -3, 2))),
+    ("axisZ", dict(phis=[0.0], kind=1, nu=27)),
+    ("wide", dict(phis=[90.0], X=70, Y=66, Z=20, nu=83, nv=90, views=21)),
+]
+
+
+@pytest.mark.parametrize("name,kw", STREAM_GEOMS, ids=[g[0] for g in STREAM_GEOMS])
+def test_stream_fp_columns_per_wave_give_identical_sums(mtr, orc, name, kw):
+    # the streaming FP loads each voxel row once for G adjacent detector
+    # columns; every column keeps its own tap order, so G = 1, 2, 4 agree
+    # bit for bit (through SIRT, which back-projects the residual)
+    pb = problem(mtr, **kw)
+    say("stream G", name)
+    outs = []
+    old = os.environ.get("MTR_FP_COLS")
+    try:
+        for g in ("1", "2", "4"):
+            os.environ["MTR_FP_COLS"] = g
+            with load_ctx(mtr, pb) as c:
+                assert c.agent_kernels(0)[0] == mtr.FPK_STREAM
+                c.bpt(0, out=False)
+                x = c.sirt(0, 2, 0.5)
+                outs.append((x, c.residual_norm2(0)))
+    finally:
+        if old is None:
+            os.environ.pop("MTR_FP_COLS", None)
+        else:
+            os.environ["MTR_FP_COLS"] = old
+    for x, r2 in outs[1:]:
+        assert np.array_equal(x, outs[0][0]) and r2 == outs[0][1]
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 2, 0.5)
+    assert orc.rel_l2(outs[0][0], ref) < SOLVE_TOL
