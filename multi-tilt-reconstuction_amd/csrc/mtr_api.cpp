@@ -324,7 +324,7 @@ struct Agent {
     bool lanes_k = false;  // forward projection with lanes along detector rows
     BpMode bpm;            // back-projection kernel choice
     int fp_nc = 0;         // > 0: streaming forward projection with this candidate count
-    int fp_g = 1;          // streaming forward projection: detector columns per wave (1, 2, 4)
+    int fp_g = 1;          // streaming forward projection: detector columns per wave (1, 2)
     int npartial = 0;      // per-block partials written by the last FP launch
     int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
     int64_t st[3] = {0, 0, 0};
@@ -943,7 +943,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     // the streaming FP addresses the volume with 32-bit byte offsets
     if ((uint64_t)ctx->vol.X * ctx->vol.Y * ctx->vol.Z * elem_size(ctx->precision) > 0xFFFFFFFFull) ag.fp_nc = 0;
     ag.fp_g = 2;
-    if (const char *g = std::getenv("MTR_FP_COLS")) ag.fp_g = std::atoi(g) == 4 ? 4 : (std::atoi(g) == 1 ? 1 : 2);
+    if (const char *g = std::getenv("MTR_FP_COLS")) ag.fp_g = std::atoi(g) == 1 ? 1 : 2;
     ag.fast = ag.fp_nc ? sW : 0;
     if (!ag.fp_nc && !ag.bpm.sep) {
         ag.col = col_mode(xc, yc, A, *prj, 16);

@@ -397,21 +397,30 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
 // Block: 4 waves, each G adjacent detector columns i0 .. i0 + G - 1 of one
 // view; the 64 lanes are 64 consecutive detector rows k, i.e. 64 adjacent
 // planes W(k).  All lanes share the ray paths, so the support is generated
-// once per wave: lane q builds candidate voxel q of the current chunk (the
-// union over the G columns of each line's lattice points), a ballot keeps
-// the voxels with a nonzero weight for any of the columns, and their byte
-// offsets and G weights go to the wave's LDS list; each listed voxel is
-// then one coalesced 256-byte load and G FMAs for 64 x G pixels.  A voxel
-// row that the per-column scheme loads once per column it touches (about
-// twice) is loaded once per wave, which is what bounds this kernel: the
-// vector-memory data path.  Every column still sums its own taps in (line,
-// candidate) order, the order of the per-lane path, and the weights are the
-// same expressions, so the sums are identical for every G (the extra FMAs
-// add exact zeros).
+// once per wave, 64 lines at a time: lane q takes line l0 + q, walks the
+// union over the G columns of that line's lattice points and keeps the
+// voxels with a nonzero weight for any column; a wave prefix sum of the
+// per-line counts places them, in (line, point) order, in the wave's LDS
+// list as byte offsets and G weights.  Each listed voxel is then one
+// coalesced 256-byte load and G FMAs for 64 x G pixels, with the loads of
+// the next 8 voxels in flight.  A voxel row that a one-column scheme loads
+// once per column it touches (about twice) is loaded once per wave; the
+// vector-memory data path is what bounds this kernel.  Every column still
+// sums its own taps in (line, point) order, the order of the per-lane path,
+// with the same weight expressions, so the sums are those of the per-lane
+// path for every G (the extra FMAs add exact zeros).
+// Binary digits of n (n >= 0).
+__host__ __device__ constexpr int bits_for(int n) { return n < 2 ? 1 : 1 + bits_for(n / 2); }
+
 template <class T, int NC, int G>
 __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ ags)
 {
-    constexpr int TP = 72;  // per wave: 64 listed voxels + 8 padding
+    // lattice points per line and column pair: the columns' first points
+    // lie (G - 1) |mi| < 2 (G - 1) apart (|mi| = Em < 2 when NC <= 4), plus
+    // one against rounding of the starts
+    constexpr int NU = NC + 2 * (G - 1) + 1;
+    constexpr int TP = 64 * NU + 8;  // per wave: 64 lines' voxels + 8 padding
+    constexpr int CB = bits_for(NU);  // binary digits of a line's count (0 .. NU)
     __shared__ int tap_off[4][TP];
     __shared__ float tap_w[4][G][TP];
     const AgentDev &ag = ags[blockIdx.y];
@@ -445,9 +454,10 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const int64_t sL = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.L), sM = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.M),
                   sW = sel3<int64_t>(ag.sx, ag.sy, ag.sz, pl.W);
     const int64_t base0 = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy - (int64_t)ag.Zs * ag.sz;
-    // per-lane 32-bit byte offset of the lane's plane (the volume is below
-    // 4 GiB: the host selects this kernel only then)
+    // 32-bit byte offsets (the volume is below 4 GiB: the host selects this
+    // kernel only then): the lane's plane, and steps along L and M
     const uint32_t lane_off = (uint32_t)(((row_ok ? w : Wlo) - Wlo) * sW) * (uint32_t)sizeof(T);
+    const uint32_t bL = (uint32_t)sL * (uint32_t)sizeof(T), bM = (uint32_t)sM * (uint32_t)sizeof(T);
     float acc[G];
 #pragma unroll
     for (int g = 0; g < G; g++) acc[g] = 0.f;
@@ -475,102 +485,106 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
                 l1 = max(l1, min(Lhi, (int)ceil(lhi)));
             }
         }
-        if (l0 <= l1) {
-            const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
-            const unsigned mspan = (unsigned)(Mhi - Mlo);
-            // candidate slots per line: the columns' lattice points start
-            // (G - 1) |mi| apart at most (one more slot where that spread is
-            // an integer, against rounding of the starts)
-            int ncu = NC;
-            if (G > 1) {
-                const double spread = (G - 1) * fabs(pl.mi);
-                ncu += (int)ceil(spread) + (fabs(spread - rint(spread)) < 1e-6 ? 1 : 0);
+        const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
+        const unsigned mspan = (unsigned)(Mhi - Mlo);
+        // scalar base: voxel (Llo, Mlo) of the box's first plane
+        const auto xc = uniform_base(x + base0 + (int64_t)Wlo * sW + (int64_t)Llo * sL + (int64_t)Mlo * sM);
+        for (int lb = l0; lb <= l1; lb += 64) {
+            // this lane's line and its lattice points mb .. mb + NU - 1
+            const int l = lb + lane;
+            double mc[G], t0[G];
+            int mfi[G];
+            int mb = 0;
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                mc[g] = ms[g] + l * pl.ml;
+                const double mf = floor(mc[g] - pl.Em) + 1.0;
+                mfi[g] = (int)mf;
+                t0[g] = aM * (mf - mc[g]);
+                mb = g == 0 ? mfi[0] : min(mb, mfi[g]);
             }
-            const int n = (l1 - l0 + 1) * ncu;
-            // scalar base: voxel (Llo, Mlo) of the box's first plane
-            const auto xc = uniform_base(x + base0 + (int64_t)Wlo * sW + (int64_t)Llo * sL + (int64_t)Mlo * sM);
-            for (int c0 = 0; c0 < n; c0 += 64) {
-                // candidate c: line l0 + c / ncu, voxel mb + c % ncu
-                const int c = c0 + lane;
-                const int l = l0 + c / ncu, j = c - (c / ncu) * ncu;
-                double mc[G], mf[G];
-                double mb = 0.0;
+            // weight of column g at point m: point m - mf of its line, the
+            // same expression as the per-lane path
+            float wt[NU][G];
+            uint32_t keep = 0;
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < NU; j++) {
+                const int m = mb + j;
+                const bool ok = l <= l1 && (unsigned)(m - Mlo) <= mspan;
+                bool any = false;
 #pragma unroll
                 for (int g = 0; g < G; g++) {
-                    mc[g] = ms[g] + l * pl.ml;
-                    mf[g] = floor(mc[g] - pl.Em) + 1.0;
-                    mb = g == 0 ? mf[0] : fmin(mb, mf[g]);
+                    const int jg = m - mfi[g];
+                    wt[j][g] = ok && g < gn && (unsigned)jg < (unsigned)NC ? tent(fma((double)jg, aM, t0[g])) : 0.0f;
+                    any = any || wt[j][g] > 0.0f;
                 }
-                const int m = (int)mb + j;
-                const bool ok = c < n && (unsigned)(m - Mlo) <= mspan;
-                // weight of column g: point m - mf[g] of its line, the same
-                // expression as the per-lane path
-                float wt[G];
-                bool anyw = false;
-#pragma unroll
-                for (int g = 0; g < G; g++) {
-                    const int jg = m - (int)mf[g];
-                    wt[g] = ok && g < gn && jg >= 0 && jg < NC ? tent(fma((double)jg, aM, aM * (mf[g] - mc[g])))
-                                                                 : 0.0f;
-                    anyw = anyw || wt[g] > 0.0f;
-                }
-                // byte offset from voxel (Llo, Mlo) of the first plane, >= 0
-                const int off = ok ? (int)((l - Llo) * sL + (m - Mlo) * sM) * (int)sizeof(T) : 0;
-                // compact the listed voxels of this chunk into the wave's LDS
-                // list (order preserved), padded to a multiple of 8
-                const uint64_t mask = __ballot(anyw);
-                const int cnt = __popcll(mask);
-                const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-                if (anyw) {
-                    tap_off[wid][pos] = off;
-#pragma unroll
-                    for (int g = 0; g < G; g++) tap_w[wid][g][pos] = wt[g];
-                }
-                const int pad = lane >= cnt && lane < cnt + 8 ? lane : (lane < 8 && 64 + lane < cnt + 8 ? 64 + lane : -1);
-                if (pad >= 0) {
-                    tap_off[wid][pad] = 0;
-#pragma unroll
-                    for (int g = 0; g < G; g++) tap_w[wid][g][pad] = 0.0f;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                // voxels in groups of 8 held in two register sets: the next
-                // group's loads are in flight while the current one is summed
-                const int cnt8 = (cnt + 7) & ~7;
-                float xa[8], xb[8];
-                auto fetch = [&](int e, float(&xv)[8]) {
-#pragma unroll
-                    for (int q = 0; q < 8; q++) xv[q] = ldb<T>(xc, (uint32_t)tap_off[wid][e + q] + lane_off);
-                };
-                auto sum = [&](int e, const float(&xv)[8]) {
-#pragma unroll
-                    for (int q = 0; q < 8; q++)
-#pragma unroll
-                        for (int g = 0; g < G; g++) acc[g] = fmaf(tap_w[wid][g][e + q], xv[q], acc[g]);
-                };
-                if (cnt > 0) {
-                    fetch(0, xa);
-                    int e = 0;
-                    for (; e + 16 < cnt8; e += 16) {
-                        fetch(e + 8, xb);
-                        sum(e, xa);
-                        fetch(e + 16, xa);
-                        sum(e + 8, xb);
-                    }
-                    if (e + 8 < cnt8) {
-                        fetch(e + 8, xb);
-                        sum(e, xa);
-                        sum(e + 8, xb);
-                    } else {
-                        sum(e, xa);
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                keep |= any ? 1u << j : 0u;
+                cnt += any ? 1 : 0;
             }
+            // exclusive prefix sum of the per-line counts over the wave, bit
+            // plane by bit plane
+            const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            int pos = 0, total = 0;
+#pragma unroll
+            for (int q = 0; q < CB; q++) {
+                const uint64_t mq = __ballot((cnt >> q) & 1);
+                pos += __popcll(mq & below) << q;
+                total += __popcll(mq) << q;
+            }
+            const uint32_t off0 = (uint32_t)(l - Llo) * bL + (uint32_t)(mb - Mlo) * bM;
+#pragma unroll
+            for (int j = 0; j < NU; j++) {
+                if (keep & (1u << j)) {
+                    tap_off[wid][pos] = (int)(off0 + (uint32_t)j * bM);
+#pragma unroll
+                    for (int g = 0; g < G; g++) tap_w[wid][g][pos] = wt[j][g];
+                    pos++;
+                }
+            }
+            if (lane < 8) {
+                // padding to a multiple of 8: voxel (Llo, Mlo), weight 0
+                tap_off[wid][total + lane] = 0;
+#pragma unroll
+                for (int g = 0; g < G; g++) tap_w[wid][g][total + lane] = 0.0f;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // voxels in groups of 8 held in two register sets: the next
+            // group's loads are in flight while the current one is summed
+            const int cnt8 = (total + 7) & ~7;
+            float xa[8], xb[8];
+            auto fetch = [&](int e, float(&xv)[8]) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) xv[q] = ldb<T>(xc, (uint32_t)tap_off[wid][e + q] + lane_off);
+            };
+            auto sum = [&](int e, const float(&xv)[8]) {
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+#pragma unroll
+                    for (int g = 0; g < G; g++) acc[g] = fmaf(tap_w[wid][g][e + q], xv[q], acc[g]);
+            };
+            if (total > 0) {
+                fetch(0, xa);
+                int e = 0;
+                for (; e + 16 < cnt8; e += 16) {
+                    fetch(e + 8, xb);
+                    sum(e, xa);
+                    fetch(e + 16, xa);
+                    sum(e + 8, xb);
+                }
+                if (e + 8 < cnt8) {
+                    fetch(e + 8, xb);
+                    sum(e, xa);
+                    sum(e + 8, xb);
+                } else {
+                    sum(e, xa);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     double r2 = 0.0;
@@ -2049,7 +2063,6 @@ static hipError_t launch_fp_stream_g(const AgentDev *d_ags, int nag, int max_blo
 template <class T>
 static hipError_t launch_fp_stream_t(const AgentDev *d_ags, int nag, int max_blocks, int nc, int g, hipStream_t st)
 {
-    if (g == 4) return launch_fp_stream_g<T, 4>(d_ags, nag, max_blocks, nc, st);
     if (g == 2) return launch_fp_stream_g<T, 2>(d_ags, nag, max_blocks, nc, st);
     return launch_fp_stream_g<T, 1>(d_ags, nag, max_blocks, nc, st);
 }
