@@ -788,6 +788,35 @@ struct BpSepStage {
 
 constexpr int kSepWaves = 8;  // waves along P2 per block (512 threads)
 
+static_assert(sizeof(BpSepView) == 48 && offsetof(BpSepView, k0) == 40, "BpSepView: five doubles, two ints");
+
+// u coefficients c[0..3] of a view through the scalar cache.
+__device__ __forceinline__ void load_ucoef(const ViewCoef *tab, int a, double (&c)[4])
+{
+    const __attribute__((address_space(4))) double *d =
+        (const __attribute__((address_space(4))) double *)(tab + a);
+#pragma unroll
+    for (int q = 0; q < 4; q++) c[q] = d[q];
+}
+
+// View constants through the scalar cache (the table is read-only during a
+// launch), so that waiting for them never waits for vector loads in flight.
+__device__ __forceinline__ BpSepView load_bsv(const BpSepView *tab, int a)
+{
+    const __attribute__((address_space(4))) double *d =
+        (const __attribute__((address_space(4))) double *)(tab + a);
+    const __attribute__((address_space(4))) int *k = (const __attribute__((address_space(4))) int *)(d + 5);
+    BpSepView v;
+    v.c0 = d[0];
+    v.a1 = d[1];
+    v.a2 = d[2];
+    v.umin = d[3];
+    v.umax = d[4];
+    v.k0 = k[0];
+    v.kstep = k[1];
+    return v;
+}
+
 template <class T, int RW, int PITCH, bool BPT>
 __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda,
                                                           BpSepArgs sa)
@@ -843,8 +872,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     // full runs: every staged row is on the detector (host-checked)
     const bool full_run = ew == RW - 1;
 
-    auto fetch = [&](int a, int &fi0, int &width, int &k0row, int &kstep) {
-        const BpSepView bv = ag.bsv[a];
+    auto fetch = [&](const BpSepView &bv, int a, int &fi0, int &width, int &k0row, int &kstep) {
         const double u0 = fma(bv.a1, (double)P10, fma(bv.a2, (double)P20, bv.c0));
         fi0 = max((int)floor(u0 + bv.umin) - 1, -1);
         const int fi1 = min((int)floor(u0 + bv.umax) + 2, nu);
@@ -884,13 +912,17 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     };
 
     int fi0, width, k0row, kstep;
-    fetch(0, fi0, width, k0row, kstep);
+    BpSepView bvn = load_bsv(ag.bsv, 0);
+    fetch(bvn, 0, fi0, width, k0row, kstep);
     commit(0);
     __syncthreads();
     for (int a = 0; a < ag.A; a++) {
         const int cfi0 = fi0, cwidth = width;
-        if (a + 1 < ag.A) fetch(a + 1, fi0, width, k0row, kstep);
-        const BpSepView bv = ag.bsv[a];
+        const BpSepView bv = bvn;
+        if (a + 1 < ag.A) {
+            bvn = load_bsv(ag.bsv, a + 1);
+            fetch(bvn, a + 1, fi0, width, k0row, kstep);
+        }
         const double u = fma(bv.a1, (double)p1, fma(bv.a2, (double)p2, bv.c0));
         const double fu = floor(u);
         const int i0 = (int)fu;
@@ -1500,10 +1532,11 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     // detector columns of the tile in view a: u at the tile origin plus
     // the host's extreme offsets over the tile, one column of margin
     const int Xt = p1y ? P20 : P10, Yt = p1y ? P10 : P20;
-    auto window = [&](int a, int &fi0, int &fi1) {
-        const BpSepView bv = ag.bsv[a];
-        const ViewCoef vc = ag.coef[a];
-        const double ut = fma(vc.c[3], (double)Z0, fma(vc.c[2], (double)Yt, fma(vc.c[1], (double)Xt, vc.c[0]))) - 1.0;
+    // (view constants come through the scalar cache, so that waiting for
+    // them never waits for the staging loads in flight)
+    auto window = [&](int a, const double (&uc)[4], int &fi0, int &fi1) {
+        const BpSepView bv = load_bsv(ag.bsv, a);
+        const double ut = fma(uc[3], (double)Z0, fma(uc[2], (double)Yt, fma(uc[1], (double)Xt, uc[0]))) - 1.0;
         fi0 = (int)floor(ut + bv.umin) - 1;
         fi1 = (int)floor(ut + bv.umax) + 1;
     };
@@ -1546,7 +1579,9 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     };
 
     int fi0, fi1;
-    window(0, fi0, fi1);
+    double ucn[4];
+    load_ucoef(ag.coef, 0, ucn);
+    window(0, ucn, fi0, fi1);
     if constexpr (!NORM) {
         fetch(0, fi0, fi1);
         commit(0);
@@ -1554,14 +1589,15 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     }
     for (int a = 0; a < ag.A; a++) {
         const int cfi0 = fi0, cfi1 = fi1;
+        const double uc[4] = {ucn[0], ucn[1], ucn[2], ucn[3]};
         if (a + 1 < ag.A) {
-            window(a + 1, fi0, fi1);
+            load_ucoef(ag.coef, a + 1, ucn);
+            window(a + 1, ucn, fi0, fi1);
             if constexpr (!NORM) fetch(a + 1, fi0, fi1);
         }
-        const ViewCoef vc = ag.coef[a];
-        const double ub = fma(vc.c[3], (double)Z32, fma(vc.c[2], (double)Y, fma(vc.c[1], (double)X, vc.c[0]))) - 1.0;
+        const double ub = fma(uc[3], (double)Z32, fma(uc[2], (double)Y, fma(uc[1], (double)X, uc[0]))) - 1.0;
         const double fb = floor(ub);
-        const float f0 = (float)(ub - fb), step = (float)vc.c[3];
+        const float f0 = (float)(ub - fb), step = (float)uc[3];
         const int ib = (int)fb - cfi0;
         const float2 *row0 = lds_raw + (a & 1) * kColElems + lr;
         const float2 *row1 = row0 + pitch;
