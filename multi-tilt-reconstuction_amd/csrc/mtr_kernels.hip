@@ -859,7 +859,10 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     // Consecutive threads take consecutive columns in the [nv][nu] layout
     // and consecutive rows in the row-fastest [nu][nv] layout, so the
     // staging loads are contiguous either way.
-    const bool rowfast = ag.pk == 1;
+    // agent constants used in the view loop, held in registers
+    const int pk = ag.pk, pi = ag.pi, nviews = ag.A;
+    const BpSepView *const bsv = ag.bsv;
+    const bool rowfast = pk == 1;
     int srow[St::kPer], scol[St::kPer], coff[St::kPer];
 #pragma unroll
     for (int q = 0; q < St::kPer; q++) {
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         const int r = rowfast ? e % RW : e / PITCH, cidx = rowfast ? e / RW : e % PITCH;
         srow[q] = e < St::kElems ? r : RW;  // RW: no slot
         scol[q] = cidx;
-        coff[q] = cidx * ag.pi;
+        coff[q] = cidx * pi;
     }
     // full runs: every staged row is on the detector (host-checked)
     const bool full_run = ew == RW - 1;
@@ -882,12 +885,12 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         const uint32_t vbase = (uint32_t)(plane * a);
         if (full_run && fi0 >= 0 && fi0 + PITCH <= nu - 1) {
             // every slot's pair lies inside the detector: no clamps or masks
-            const int base = (int)vbase + k0row * ag.pk + fi0 * ag.pi, rs = kstep * ag.pk;
+            const int base = (int)vbase + k0row * pk + fi0 * pi, rs = kstep * pk;
 #pragma unroll
             for (int q = 0; q < St::kPer; q++) {
                 const int o = base + (srow[q] < RW ? srow[q] : 0) * rs + coff[q];
                 const float v0 = ldg<T>(src, (uint32_t)o);
-                const float v1 = ldg<T>(src, (uint32_t)(o + ag.pi));
+                const float v1 = ldg<T>(src, (uint32_t)(o + pi));
                 pre[q] = make_float2(v0, v1 - v0);
             }
             return;
@@ -898,9 +901,9 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
             const int gk = k0row + kstep * srow[q], gi = fi0 + scol[q];
             const bool ok = srow[q] <= ew && scol[q] < width;
             const bool ok0 = ok && (unsigned)gi < (unsigned)nu, ok1 = ok && (unsigned)(gi + 1) < (unsigned)nu;
-            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * (uint32_t)ag.pk;
-            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1) * (uint32_t)ag.pi);
-            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1) * (uint32_t)ag.pi);
+            const uint32_t o = vbase + (uint32_t)min(max(gk, 0), nv - 1) * (uint32_t)pk;
+            const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1) * (uint32_t)pi);
+            const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1) * (uint32_t)pi);
             const float r0 = ok0 ? v0 : 0.f, r1 = ok1 ? v1 : 0.f;
             pre[q] = make_float2(r0, r1 - r0);
         }
@@ -912,15 +915,15 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     };
 
     int fi0, width, k0row, kstep;
-    BpSepView bvn = load_bsv(ag.bsv, 0);
+    BpSepView bvn = load_bsv(bsv, 0);
     fetch(bvn, 0, fi0, width, k0row, kstep);
     commit(0);
     __syncthreads();
-    for (int a = 0; a < ag.A; a++) {
+    for (int a = 0; a < nviews; a++) {
         const int cfi0 = fi0, cwidth = width;
         const BpSepView bv = bvn;
-        if (a + 1 < ag.A) {
-            bvn = load_bsv(ag.bsv, a + 1);
+        if (a + 1 < nviews) {
+            bvn = load_bsv(bsv, a + 1);
             fetch(bvn, a + 1, fi0, width, k0row, kstep);
         }
         const double u = fma(bv.a1, (double)p1, fma(bv.a2, (double)p2, bv.c0));
@@ -949,7 +952,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
             }
             Cacc += wu0 + wu1;
         }
-        if (a + 1 < ag.A) commit((a + 1) & 1);
+        if (a + 1 < nviews) commit((a + 1) & 1);
         __syncthreads();
     }
     // Epilogue: each lane holds a run along W (contiguous in memory) at its
