@@ -863,15 +863,23 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     const int pk = ag.pk, pi = ag.pi, nviews = ag.A;
     const BpSepView *const bsv = ag.bsv;
     const bool rowfast = pk == 1;
-    int srow[St::kPer], scol[St::kPer], coff[St::kPer];
+    int srow[St::kPer], scol[St::kPer];
+    // byte offsets of a slot's pair within a footprint whose lowest row and
+    // first column are at the origin: rows ascending with W (kstep > 0) or
+    // descending (kstep < 0), so that both stay non-negative
+    uint32_t oup[St::kPer], odn[St::kPer];
 #pragma unroll
     for (int q = 0; q < St::kPer; q++) {
         const int e = threadIdx.x + NT * q;
         const int r = rowfast ? e % RW : e / PITCH, cidx = rowfast ? e / RW : e % PITCH;
         srow[q] = e < St::kElems ? r : RW;  // RW: no slot
         scol[q] = cidx;
-        coff[q] = cidx * pi;
+        const int rr = e < St::kElems ? r : 0;
+        oup[q] = (uint32_t)(rr * pk + cidx * pi) * (uint32_t)sizeof(T);
+        odn[q] = (uint32_t)((RW - 1 - rr) * pk + cidx * pi) * (uint32_t)sizeof(T);
     }
+    const uint32_t pib = (uint32_t)pi * (uint32_t)sizeof(T);
+    const T *const srcp = reinterpret_cast<const T *>(BPT ? ag.p : ag.r);
     // full runs: every staged row is on the detector (host-checked)
     const bool full_run = ew == RW - 1;
 
@@ -884,13 +892,16 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         kstep = bv.kstep;
         const uint32_t vbase = (uint32_t)(plane * a);
         if (full_run && fi0 >= 0 && fi0 + PITCH <= nu - 1) {
-            // every slot's pair lies inside the detector: no clamps or masks
-            const int base = (int)vbase + k0row * pk + fi0 * pi, rs = kstep * pk;
+            // every slot's pair lies inside the detector: no clamps or masks;
+            // a scalar base at the footprint's lowest row and first column
+            const bool up = kstep > 0;
+            const int krow = up ? k0row : k0row - (RW - 1);
+            const auto sp = uniform_base(srcp + (int64_t)plane * a + (int64_t)krow * pk + (int64_t)fi0 * pi);
 #pragma unroll
             for (int q = 0; q < St::kPer; q++) {
-                const int o = base + (srow[q] < RW ? srow[q] : 0) * rs + coff[q];
-                const float v0 = ldg<T>(src, (uint32_t)o);
-                const float v1 = ldg<T>(src, (uint32_t)(o + pi));
+                const uint32_t o = up ? oup[q] : odn[q];
+                const float v0 = ldb<T>(sp, o);
+                const float v1 = ldb<T>(sp, o + pib);
                 pre[q] = make_float2(v0, v1 - v0);
             }
             return;
@@ -933,24 +944,24 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
         const int li = min(max(i0 - cfi0, 0), min(cwidth, PITCH) - 2);
         const float2 *q = &lds[a & 1][li];
         const bool interior = cfi0 >= 0 && cfi0 + cwidth - 1 <= nu - 1;
-        if (cwidth < 2) {
-            // the view misses this tile: no tap is on the detector
-        } else if (interior) {
+        // (a view with cwidth < 2 misses this tile: no tap is on the detector)
+        if (cwidth >= 2) {
+            // S += wA r0 + wB (r1 - r0): wA = 1, wB = du inside the detector
+            // (fma by 1 is the plain sum), at its side edges the taps' weights
+            // with the off-detector one at 0
+            float wA = 1.0f, wB = du;
+            if (!interior) {
+                const float wu0 = (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
+                const float wu1 = (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
+                wA = wu0 + wu1;
+                wB = wu1;
+            }
 #pragma unroll
             for (int t = 0; t < RW; t++) {
                 const float2 rd = q[t * PITCH];
-                S[t] = fmaf(du, rd.y, S[t] + rd.x);
+                S[t] = fmaf(wB, rd.y, fmaf(wA, rd.x, S[t]));
             }
-            Cacc += 1.0f;
-        } else {
-            const float wu0 = (unsigned)i0 < (unsigned)nu ? 1.f - du : 0.f;
-            const float wu1 = (unsigned)(i0 + 1) < (unsigned)nu ? du : 0.f;
-#pragma unroll
-            for (int t = 0; t < RW; t++) {
-                const float2 rd = q[t * PITCH];
-                S[t] += wu0 * rd.x + wu1 * (rd.x + rd.y);
-            }
-            Cacc += wu0 + wu1;
+            Cacc += wA;
         }
         if (a + 1 < nviews) commit((a + 1) & 1);
         __syncthreads();
