@@ -817,14 +817,15 @@ __device__ __forceinline__ BpSepView load_bsv(const BpSepView *tab, int a)
     return v;
 }
 
-template <class T, int RW, int PITCH, bool BPT>
+template <class T, int RW, int PITCH, bool BPT, int VPS>
 __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda,
                                                           BpSepArgs sa)
 {
     constexpr int NT = 64 * kSepWaves;
     // LDS holds (r[k][i], r[k][i+1] - r[k][i]) pairs so that the interior
     // update is S += r0 + du * d: one 64-bit LDS read and two FP32 ops.
-    constexpr int kStage = 2 * RW * PITCH, kEpi = kSepWaves * 64 * 18 / 2;
+    // VPS views are staged per barrier, in two buffers.
+    constexpr int kStage = 2 * VPS * RW * PITCH, kEpi = kSepWaves * 64 * 18 / 2;
     __shared__ float2 lds_raw[kStage > kEpi ? kStage : kEpi];
     float2(*lds)[RW * PITCH] = reinterpret_cast<float2(*)[RW * PITCH]>(lds_raw);
     const AgentDev &ag = ags[blockIdx.y];
@@ -854,7 +855,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
 #pragma unroll
     for (int t = 0; t < RW; t++) S[t] = 0.f;
     float Cacc = 0.f;  // every row of the run is on the detector (host-checked)
-    float2 pre[St::kPer];
+    float2 pre[VPS][St::kPer];
     // this thread's staging slots: fixed (row, column) of the footprint.
     // Consecutive threads take consecutive columns in the [nv][nu] layout
     // and consecutive rows in the row-fastest [nu][nv] layout, so the
@@ -883,13 +884,12 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     // full runs: every staged row is on the detector (host-checked)
     const bool full_run = ew == RW - 1;
 
-    auto fetch = [&](const BpSepView &bv, int a, int &fi0, int &width, int &k0row, int &kstep) {
+    auto fetch = [&](const BpSepView &bv, int a, float2(&pv)[St::kPer], int &fi0, int &width) {
         const double u0 = fma(bv.a1, (double)P10, fma(bv.a2, (double)P20, bv.c0));
         fi0 = max((int)floor(u0 + bv.umin) - 1, -1);
         const int fi1 = min((int)floor(u0 + bv.umax) + 2, nu);
         width = fi1 - fi0 + 1;
-        k0row = bv.k0 + bv.kstep * W0;
-        kstep = bv.kstep;
+        const int k0row = bv.k0 + bv.kstep * W0, kstep = bv.kstep;
         const uint32_t vbase = (uint32_t)(plane * a);
         if (full_run && fi0 >= 0 && fi0 + PITCH <= nu - 1) {
             // every slot's pair lies inside the detector: no clamps or masks;
@@ -902,7 +902,7 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
                 const uint32_t o = up ? oup[q] : odn[q];
                 const float v0 = ldb<T>(sp, o);
                 const float v1 = ldb<T>(sp, o + pib);
-                pre[q] = make_float2(v0, v1 - v0);
+                pv[q] = make_float2(v0, v1 - v0);
             }
             return;
         }
@@ -916,33 +916,22 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
             const float v0 = ldg<T>(src, o + (uint32_t)min(max(gi, 0), nu - 1) * (uint32_t)pi);
             const float v1 = ldg<T>(src, o + (uint32_t)min(max(gi + 1, 0), nu - 1) * (uint32_t)pi);
             const float r0 = ok0 ? v0 : 0.f, r1 = ok1 ? v1 : 0.f;
-            pre[q] = make_float2(r0, r1 - r0);
+            pv[q] = make_float2(r0, r1 - r0);
         }
     };
-    auto commit = [&](int buf) {
+    auto commit = [&](int slot, const float2(&pv)[St::kPer]) {
 #pragma unroll
         for (int q = 0; q < St::kPer; q++)
-            if (srow[q] < RW) lds[buf][srow[q] * PITCH + scol[q]] = pre[q];
+            if (srow[q] < RW) lds[slot][srow[q] * PITCH + scol[q]] = pv[q];
     };
-
-    int fi0, width, k0row, kstep;
-    BpSepView bvn = load_bsv(bsv, 0);
-    fetch(bvn, 0, fi0, width, k0row, kstep);
-    commit(0);
-    __syncthreads();
-    for (int a = 0; a < nviews; a++) {
-        const int cfi0 = fi0, cwidth = width;
-        const BpSepView bv = bvn;
-        if (a + 1 < nviews) {
-            bvn = load_bsv(bsv, a + 1);
-            fetch(bvn, a + 1, fi0, width, k0row, kstep);
-        }
+    // one view's taps from its staged footprint
+    auto gather = [&](const BpSepView &bv, int cfi0, int cwidth, const float2 *rows) {
         const double u = fma(bv.a1, (double)p1, fma(bv.a2, (double)p2, bv.c0));
         const double fu = floor(u);
         const int i0 = (int)fu;
         const float du = (float)(u - fu);
         const int li = min(max(i0 - cfi0, 0), min(cwidth, PITCH) - 2);
-        const float2 *q = &lds[a & 1][li];
+        const float2 *q = rows + li;
         const bool interior = cfi0 >= 0 && cfi0 + cwidth - 1 <= nu - 1;
         // (a view with cwidth < 2 misses this tile: no tap is on the detector)
         if (cwidth >= 2) {
@@ -963,7 +952,46 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
             }
             Cacc += wA;
         }
-        if (a + 1 < nviews) commit((a + 1) & 1);
+    };
+
+    // stage s holds views s VPS .. s VPS + VPS - 1 in buffer s & 1; the next
+    // stage's loads are in flight while the current one is gathered, views
+    // in order
+    int fi0[VPS], width[VPS];
+    BpSepView bvs[VPS];
+#pragma unroll
+    for (int v = 0; v < VPS; v++)
+        if (v < nviews) {
+            bvs[v] = load_bsv(bsv, v);
+            fetch(bvs[v], v, pre[v], fi0[v], width[v]);
+        }
+#pragma unroll
+    for (int v = 0; v < VPS; v++)
+        if (v < nviews) commit(v, pre[v]);
+    __syncthreads();
+    for (int a0 = 0; a0 < nviews; a0 += VPS) {
+        const int buf = (a0 / VPS) & 1;
+        int cfi0[VPS], cwidth[VPS];
+        BpSepView cbv[VPS];
+#pragma unroll
+        for (int v = 0; v < VPS; v++) {
+            cfi0[v] = fi0[v];
+            cwidth[v] = width[v];
+            cbv[v] = bvs[v];
+        }
+        const int an = a0 + VPS;
+#pragma unroll
+        for (int v = 0; v < VPS; v++)
+            if (an + v < nviews) {
+                bvs[v] = load_bsv(bsv, an + v);
+                fetch(bvs[v], an + v, pre[v], fi0[v], width[v]);
+            }
+#pragma unroll
+        for (int v = 0; v < VPS; v++)
+            if (a0 + v < nviews) gather(cbv[v], cfi0[v], cwidth[v], lds[buf * VPS + v]);
+#pragma unroll
+        for (int v = 0; v < VPS; v++)
+            if (an + v < nviews) commit((buf ^ 1) * VPS + v, pre[v]);
         __syncthreads();
     }
     // Epilogue: each lane holds a run along W (contiguous in memory) at its
@@ -2127,6 +2155,10 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
 // odd (in float2 units) so that a column of 32 rows written by 32 lanes
 // spreads over all LDS banks
 constexpr int kSepPitch = 81;
+// views staged per barrier: two for runs of up to 16 planes (the four
+// footprints then still fit three blocks per CU in LDS), one for 32
+template <int RW>
+constexpr int kSepVps = RW <= 16 ? 2 : 1;
 
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 {
@@ -2142,9 +2174,11 @@ template <class T, int RW>
 static void launch_bp_sep_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, BpSepArgs sa, hipStream_t st)
 {
     if (bpt)
-        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, true>), grid, dim3(64 * kSepWaves), 0, st, d_ags, lambda, sa);
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, true, kSepVps<RW>>), grid, dim3(64 * kSepWaves), 0, st, d_ags,
+                           lambda, sa);
     else
-        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, false>), grid, dim3(64 * kSepWaves), 0, st, d_ags, lambda, sa);
+        hipLaunchKernelGGL((k_bp_sep<T, RW, kSepPitch, false, kSepVps<RW>>), grid, dim3(64 * kSepWaves), 0, st, d_ags,
+                           lambda, sa);
 }
 
 template <class T>
