@@ -370,6 +370,10 @@ struct mtr_ctx {
     void *d_wbar = nullptr;
     std::vector<Agent> agents;
     AgentDev *d_ags = nullptr;  // table in agent order
+    // the same entries ordered by forward- and by back-projection kernel
+    // kind, so that a whole-context SIRT iteration launches each kind once
+    AgentDev *d_ags_fp = nullptr, *d_ags_bp = nullptr;
+    std::vector<int> fp_order, bp_order;
     double *d_cons_partial = nullptr;
     double *d_cons_scalar = nullptr;
     int cons_blocks = 0;
@@ -495,13 +499,32 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     return d;
 }
 
+int fp_kind_key(const Agent &a);
+int bp_kind_key(const Agent &a);
+
 int upload_table(mtr_ctx *ctx)
 {
     std::vector<AgentDev> t;
     for (auto &ag : ctx->agents) t.push_back(make_dev(ctx, ag));
     if (t.empty()) return MTR_OK;
-    HIP_TRY(hipMemcpyAsync(ctx->d_ags, t.data(), sizeof(AgentDev) * t.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
+    const int K = (int)t.size();
+    auto order = [&](int (*key)(const Agent &)) {
+        std::vector<int> o(K);
+        for (int a = 0; a < K; a++) o[a] = a;
+        std::stable_sort(o.begin(), o.end(),
+                         [&](int p, int q) { return key(ctx->agents[p]) < key(ctx->agents[q]); });
+        return o;
+    };
+    ctx->fp_order = order(fp_kind_key);
+    ctx->bp_order = order(bp_kind_key);
+    std::vector<AgentDev> tf(K), tb(K);
+    for (int i = 0; i < K; i++) {
+        tf[i] = t[ctx->fp_order[i]];
+        tb[i] = t[ctx->bp_order[i]];
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->d_ags, t.data(), sizeof(AgentDev) * K, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_ags_fp, tf.data(), sizeof(AgentDev) * K, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->d_ags_bp, tb.data(), sizeof(AgentDev) * K, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return MTR_OK;
 }
@@ -638,8 +661,19 @@ Batch ctx_batch(mtr_ctx *ctx, int first, int count)
     return B;
 }
 
+// Every agent of the context in one of the kind-sorted tables.
+Batch ctx_sorted(mtr_ctx *ctx, const std::vector<int> &order, AgentDev *d_tab)
+{
+    Batch B;
+    for (int a : order) B.e.push_back({ctx, &ctx->agents[a]});
+    B.d_tab = d_tab;
+    return B;
+}
+
 int bp_group(mtr_ctx *ctx, int first, int count, bool bpt, float lambda)
 {
+    if (first == 0 && count == (int)ctx->agents.size() && (int)ctx->bp_order.size() == count)
+        return bp_batch(ctx, ctx_sorted(ctx, ctx->bp_order, ctx->d_ags_bp), 0, count, bpt, lambda);
     return bp_batch(ctx, ctx_batch(ctx, first, count), 0, count, bpt, lambda);
 }
 
@@ -872,6 +906,19 @@ int sirt_plain(mtr_ctx *ctx, int a, int iters, float lambda)
 int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda)
 {
     for (int a : ids) MTR_TRY(ensure_normaliser(ctx, a));
+    const int K = (int)ctx->agents.size();
+    bool all = (int)ids.size() == K && (int)ctx->fp_order.size() == K;
+    for (int i = 0; all && i < K; i++) all = ids[i] == i;
+    if (all) {
+        // the whole context: each kernel kind once per iteration
+        const Batch F = ctx_sorted(ctx, ctx->fp_order, ctx->d_ags_fp);
+        const Batch Bk = ctx_sorted(ctx, ctx->bp_order, ctx->d_ags_bp);
+        for (int it = 0; it < iters; it++) {
+            MTR_TRY(fp_batch(ctx, F, 0, K));
+            MTR_TRY(bp_batch(ctx, Bk, 0, K, false, lambda));
+        }
+        return MTR_OK;
+    }
     // one batch per contiguous run of agents
     std::vector<Batch> runs;
     size_t s = 0;
@@ -1253,6 +1300,8 @@ int mtr_ctx_create(int device, mtr_ctx **out)
         return fail(MTR_ERR_HIP, "stream creation failed");
     }
     if (hipMalloc(&ctx->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&ctx->d_ags_fp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&ctx->d_ags_bp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
         hipMalloc(&ctx->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
         hipMalloc(&ctx->d_cons_scalar, sizeof(double)) != hipSuccess) {
         mtr_ctx_destroy(ctx);
@@ -1274,8 +1323,8 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
                         (void *)ag.d_bpc})
             if (q) (void)hipFree(q);
     }
-    for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar,
-                    (void *)ctx->d_s, (void *)ctx->d_cc})
+    for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,
+                    (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar, (void *)ctx->d_s, (void *)ctx->d_cc})
         if (q) (void)hipFree(q);
     for (auto &pd : ctx->pending) {
         (void)hipEventDestroy(pd.a);
