@@ -81,8 +81,12 @@ def serving(mtr, args):
     # warm-up request (first launches of every kernel), not part of the stream
     warm = mtr.synth.make_problem("warm", 64, 64, 16, 4, [0.0, 45.0, 90.0], seed=9, M=1, S=1, lam=0.5, rho=0.5)
     srv.submit_problem(warm)
-    while srv.status()["done"] < 1:
+    for _ in range(100):
+        if srv.status()["done"] >= 1:
+            break
         srv.step()
+    if srv.status()["done"] < 1:
+        raise SystemExit("serving warm-up request did not finish")
     n = len(stream)
     ids = []
     q = 0
@@ -105,7 +109,15 @@ def serving(mtr, args):
             active_sum += srv.status()["active"]
         elif q < n:
             time.sleep(min(max(stream[q][0] - now, 0.0), 0.01))
+        if steps > 100000:
+            raise SystemExit("serving stream did not drain")
     end = srv.now() - start
+    failed = 0
+    for r in ids:
+        try:
+            srv.result(r)
+        except mtr.MtrError:
+            failed += 1
     lat = []
     wait = []
     for r, (arr, _) in zip(ids, stream):
@@ -120,6 +132,7 @@ def serving(mtr, args):
                      f"{mtr.synth.SERVING['max_axes']} axes over 180 deg, 61 views, M={mtr.synth.SERVING['M']}, "
                      f"S={mtr.synth.SERVING['S']}"),
         "requests": n,
+        "failed_requests": failed,
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "mean_latency_s": float(lat.mean()),

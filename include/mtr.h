@@ -296,10 +296,12 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
 /* Admission, one MACE step of the active requests, retirement; returns
  * after the step's device work has finished. */
 int mtr_server_step(mtr_server *srv, int *completed);
+/* done (and step's completed) count finished requests, including failed
+ * ones (an admission that could not allocate, for instance). */
 int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *done, size_t *hbm_in_use);
 /* w-bar of a finished request ([Z][Y][X]); times[3] (nullable) receives
  * submission, admission and completion times.  MTR_ERR_STATE while the
- * request is pending or active. */
+ * request is pending or active; a failed request returns its error. */
 int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, double *times);
 int mtr_server_now(const mtr_server *srv, double *seconds);
 /* Per-kernel timing of the batched launches (MTR_K_*), as for a context. */

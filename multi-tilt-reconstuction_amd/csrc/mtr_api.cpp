@@ -1277,6 +1277,19 @@ int copy_volume_in(mtr_ctx *ctx, void *dev, const float *host)
     return MTR_OK;
 }
 
+// Device tables of a context: the agent table in agent order and sorted
+// by kernel kind, and the consensus partial sums.
+int alloc_tables(mtr_ctx *c)
+{
+    if (hipMalloc(&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&c->d_ags_fp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&c->d_ags_bp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        hipMalloc(&c->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
+        hipMalloc(&c->d_cons_scalar, sizeof(double)) != hipSuccess)
+        return fail(MTR_ERR_OOM, "context tables");
+    return MTR_OK;
+}
+
 }  // namespace
 
 // ======================================================================
@@ -1299,13 +1312,9 @@ int mtr_ctx_create(int device, mtr_ctx **out)
         delete ctx;
         return fail(MTR_ERR_HIP, "stream creation failed");
     }
-    if (hipMalloc(&ctx->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&ctx->d_ags_fp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&ctx->d_ags_bp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&ctx->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
-        hipMalloc(&ctx->d_cons_scalar, sizeof(double)) != hipSuccess) {
+    if (const int rc = alloc_tables(ctx); rc != MTR_OK) {
         mtr_ctx_destroy(ctx);
-        return fail(MTR_ERR_OOM, "context tables");
+        return rc;
     }
     *out = ctx;
     return MTR_OK;
@@ -2082,6 +2091,7 @@ struct ServerReq {
     mtr_ctx *ctx = nullptr;
     std::vector<float> result;
     std::string error;
+    int code = MTR_ERR_HIP;  // status of a failed request
 };
 
 }  // namespace
@@ -2112,7 +2122,7 @@ size_t request_bytes(int precision, const mtr_volume_size &vol, int K, const mtr
     const size_t es = elem_size(precision);
     const size_t N = (size_t)vol.X * vol.Y * vol.Z;
     const size_t cols = (size_t)vol.X * vol.Y;
-    size_t b = es * N + sizeof(AgentDev) * kMaxAgents + 8 * 8193;  // w-bar, tables, consensus partials
+    size_t b = es * N + 3 * sizeof(AgentDev) * kMaxAgents + 8 * 8193;  // w-bar, tables, consensus partials
     for (int a = 0; a < K; a++) {
         const size_t P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
         const size_t nfp = (size_t)((prj[a].X + 63) / 64 + 16) * ((prj[a].Y + 3) / 4 + 16) * prj[a].AngN;
@@ -2141,10 +2151,7 @@ int admit(mtr_server *srv, ServerReq *r)
     c->own_stream = false;
     c->precision = srv->precision;
     r->ctx = c;
-    if (hipMalloc(&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&c->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
-        hipMalloc(&c->d_cons_scalar, sizeof(double)) != hipSuccess)
-        return fail(MTR_ERR_OOM, "request tables");
+    MTR_TRY(alloc_tables(c));
     MTR_TRY(mtr_ctx_set_volume(c, &r->vol));
     for (size_t a = 0; a < r->prj.size(); a++) {
         int id = 0;
@@ -2264,9 +2271,14 @@ int mtr_server_step(mtr_server *srv, int *completed)
         r->t_admit = srv->now();
         const int rc = admit(srv, r);
         if (rc != MTR_OK) {
+            // finished, with its error for mtr_server_result
             r->error = g_err;
+            r->code = rc;
             r->state = REQ_FAILED;
+            r->t_done = srv->now();
             release(srv, r);
+            done++;
+            srv->n_done++;
             continue;
         }
         r->state = REQ_ACTIVE;
@@ -2359,6 +2371,7 @@ int mtr_server_step(mtr_server *srv, int *completed)
         r->t_done = srv->now();
         if (rc != MTR_OK) {
             r->error = g_err;
+            r->code = rc;
             r->state = REQ_FAILED;
         } else {
             r->state = REQ_DONE;
@@ -2394,7 +2407,7 @@ int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, dou
         times[1] = r->t_admit;
         times[2] = r->t_done;
     }
-    if (r->state == REQ_FAILED) return fail(MTR_ERR_HIP, "request %lld failed: %s", (long long)request_id, r->error.c_str());
+    if (r->state == REQ_FAILED) return fail(r->code, "request %lld failed: %s", (long long)request_id, r->error.c_str());
     if (r->state != REQ_DONE) return fail(MTR_ERR_STATE, "request %lld not finished", (long long)request_id);
     if (wbar_host) std::memcpy(wbar_host, r->result.data(), sizeof(float) * r->result.size());
     return MTR_OK;

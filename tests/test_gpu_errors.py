@@ -76,7 +76,10 @@ def test_server_rejects_nonfinite_requests_at_submission(mtr):
         assert code_of(lambda: srv.submit(pb.vol, pb.prjs, [badx], pb.ycs, pb.ps, 1, 1, 0.5, 0.5)) == ERR_NONFINITE
         assert srv.status()["pending"] == 0
         rid = srv.submit(pb.vol, pb.prjs, pb.xcs, pb.ycs, pb.ps, 1, 1, 0.5, 0.5)
-        while srv.status()["done"] < 1:
+        for _ in range(100):
+            if srv.status()["done"] >= 1:
+                break
             srv.step()
+        assert srv.status()["done"] == 1
         out, _ = srv.result(rid)
         assert np.isfinite(out).all()
