@@ -265,6 +265,28 @@ def test_mace_steps_through_api_match_solve(mtr, orc):
     assert np.array_equal(fused, stepped)
 
 
+def test_kind_sorted_batches_match_per_agent_calls(mtr):
+    # agents of alternating kernel kinds (separable, column, separable,
+    # column): the fused solve launches each kind once per iteration through
+    # the kind-sorted tables, the single calls one agent at a time
+    pb = problem(mtr, [0.0, 45.0, 90.0, 135.0], X=28, Y=24, Z=10, views=7, seed=19)
+    say("kind-sorted batches")
+    with load_ctx(mtr, pb) as c:
+        kinds = [c.agent_kernels(a)[0] for a in range(pb.K)]
+        assert kinds[0] == kinds[2] != kinds[1] == kinds[3]
+        fused = c.mace_solve(2, 2, 0.5, 0.5)
+    with load_ctx(mtr, pb) as c:
+        c.mace_init()
+        for _ in range(2):
+            for a in range(pb.K):
+                c.form_z(a)
+                c.sirt(a, 2, 0.5, out=False)
+                c.mann(a, 0.5)
+            c.mean()
+        stepped = c.get_volume(mtr.VOL_WBAR)
+    assert np.array_equal(fused, stepped)
+
+
 def test_elementwise_bit_exact(mtr, orc):
     pb = problem(mtr, [0.0, 90.0], X=16, Y=12, Z=6, views=5)
     rng = np.random.default_rng(12)
