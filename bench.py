@@ -268,7 +268,9 @@ def main() -> int:
         "bytes_per_launch": d["bytes_per_launch"],
         "avg_launch_ms": d["avg_ms"],
         "vvu_per_s": d.get("vvu_per_s"),
-        "note": "projector is VALU/LDS-bound at 61 views; HBM fraction is on algorithmic bytes",
+        "note": ("at 61 views the projectors are bound on chip, not by HBM (counters: the streaming FP on the "
+                 "vector-memory data path, TA/TD about 90%; the back-projections on VALU, LDS and barriers); "
+                 "the HBM fraction is on algorithmic bytes"),
     }
     if "consensus" in per:
         roofline["consensus_GBps"] = per["consensus"]["GBps"]
