@@ -122,6 +122,12 @@ int mtr_ctx_autotune(mtr_ctx *ctx, int on);
  * the general kernel) and the axis its 64 lanes run along (0 = X,
  * 1 = Y, 2 = Z). */
 int mtr_agent_tiles(const mtr_ctx *ctx, int agent, int *run, int *lanes_axis);
+/* Detector row slots each wave of the column-plan forward projection keeps
+ * in LDS: a window that the strip walk writes out as it moves (a power of
+ * two), or 0 for whole rows (and for agents without the column plan).
+ * Diagnostic; no counterpart in the reference.  MTR_FP_COL_WINDOW=0 in the
+ * environment when an agent is added keeps whole rows. */
+int mtr_agent_fp_window(const mtr_ctx *ctx, int agent, int *slots);
 /* The reconstruction box shared by every agent (the union box the
  * reference computes in synchronizeVol, MultiAxis.cpp:73-99).  Must be set
  * before the first agent is added; allocates w-bar. */

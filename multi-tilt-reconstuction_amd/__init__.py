@@ -124,6 +124,12 @@ class Context:
         _check(lib.mtr_agent_tiles(self._h, agent, _ct.byref(r), _ct.byref(l)), "agent_tiles")
         return r.value, l.value
 
+    def agent_fp_window(self, agent: int) -> int:
+        """LDS row slots per wave of the column-plan FP (0: whole rows)."""
+        w = _ct.c_int()
+        _check(lib.mtr_agent_fp_window(self._h, agent, _ct.byref(w)), "agent_fp_window")
+        return w.value
+
     def use_graphs(self, on: bool) -> None:
         _check(lib.mtr_ctx_use_graphs(self._h, int(on)), "use_graphs")
 

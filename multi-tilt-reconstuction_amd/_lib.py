@@ -52,6 +52,7 @@ _PROTOS = {
     "mtr_ctx_use_graphs": (_i, [_v, _i]),
     "mtr_ctx_autotune": (_i, [_v, _i]),
     "mtr_agent_tiles": (_i, [_v, _i, ct.POINTER(_i), ct.POINTER(_i)]),
+    "mtr_agent_fp_window": (_i, [_v, _i, ct.POINTER(_i)]),
     "mtr_ctx_set_volume": (_i, [_v, _pvol]),
     "mtr_agent_add": (_i, [_v, _pprj, _pd, _pd, _pf, _pi]),
     "mtr_agent_add_device": (_i, [_v, _pprj, _pd, _pd, _v, _pi]),

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mtr.h"
@@ -341,6 +342,7 @@ struct Agent {
     float *d_p_std = nullptr;  // [A][nv][nu] copy of p for the plain form (rowfast only)
     ColMode col;               // column plan (oblique in-plane axes)
     bool tuned = false;        // back-projection tiles chosen by the autotuner
+    int col_win = 0, col_jit = 0, col_span = 0;  // column FP row window (slots, power of two; 0: whole width)
     int2 *d_col_xy = nullptr;
     float2 *d_col_w = nullptr;
     int *d_strip = nullptr;
@@ -459,6 +461,9 @@ int collect_timing(mtr_ctx *ctx)
     return MTR_OK;
 }
 
+// LDS row slots per wave of the column-plan FP.
+int fp_col_slots(const Agent &ag) { return ag.col_win > 0 ? ag.col_win : ag.prj.X + 2; }
+
 AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
 {
     AgentDev d;
@@ -493,6 +498,9 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.nstrip = ag.col.on ? ag.prj.Y + 1 : 0;
     d.col_p1y = ag.col.p1y;
     d.col_pitch = ag.col.pitch;
+    d.col_win = ag.col_win;
+    d.col_jit = ag.col_jit;
+    d.col_span = ag.col_span;
     d.pm = ag.d_pm;
     d.pn = ag.d_pn;
     d.bpc = ag.d_bpc;
@@ -546,7 +554,8 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
         Timed t(ctx, MTR_K_NORMALISER);
         if (ag.col.on) {
             const AgentDev d = make_dev(ctx, ag);
-            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), ag.prj.X, true, ctx->stream));
+            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), fp_col_slots(ag), true,
+                                  ctx->stream));
             HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + a, 1, fp_col_res_blocks(d), 2, ag.d_c,
                                       ctx->stream));
             // and the back-projection normaliser of every voxel
@@ -600,7 +609,7 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
                 ag.npartial = fp_col_res_blocks(d);
                 mb = std::max(mb, fp_col_blocks(d));
                 mr = std::max(mr, ag.npartial);
-                nu = std::max(nu, ag.prj.X);
+                nu = std::max(nu, fp_col_slots(ag));
             } else {
                 ag.npartial = ag.fp_nc ? fp_stream_blocks(d, ag.fp_g) : ag.nfp;
                 mb = std::max(mb, ag.npartial);
@@ -935,6 +944,73 @@ int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda
     return MTR_OK;
 }
 
+// Row window of the column-plan FP (ColRows in the kernels).  For every
+// view and strip, the columns in walk order span slots sa .. sb (slot =
+// detector column + 1, from the kernel's u at the planes of smallest and
+// largest u).  jit is the largest drop of a column's sa below the largest
+// sa before it, span the largest sb - sa (slots past the detector do not
+// count); the window needs more than span + jit slots.  The reverse walk is
+// chosen when it needs a smaller window (u falling along the memory order).
+// win = 0 when whole rows (L slots) already leave room for six blocks per
+// CU in its 160 KiB of LDS (the window's bookkeeping then costs more than the
+// extra waves gain: 4.29 -> 4.43 ms at L = 770), or when no window smaller
+// than L suffices; rev: walk each strip backwards.
+void col_window(const std::vector<int2> &xy, const std::vector<int> &cnt, const double *xc, int A,
+                const mtr_volume_size &vol, int nu, int *win, int *jit, int *span, bool *rev)
+{
+    const int L = nu + 2, ns = (int)cnt.size() - 1;
+    *win = 0;
+    *rev = false;
+    if ((160 << 10) / (fp_col_waves() * L * (int)sizeof(float2)) >= 6) return;
+    const int zs = vol.Zstart, nz = vol.Z;
+    // views [v0, v1): (largest drop, largest span)
+    auto pass = [&](bool back, int v0, int v1) {
+        std::pair<int, int> out{0, 0};
+        for (int v = v0; v < v1; v++) {
+            const double *c = xc + 4 * v;
+            const double zlo = c[3] >= 0.0 ? (double)zs : (double)(zs + nz - 1);
+            const double zhi = c[3] >= 0.0 ? (double)(zs + nz - 1) : (double)zs;
+            for (int s = 0; s < ns; s++) {
+                int runmax = -0x40000000;
+                const int n = cnt[s + 1] - cnt[s];
+                for (int q = 0; q < n; q++) {
+                    const int2 p = xy[back ? cnt[s + 1] - 1 - q : cnt[s] + q];
+                    const double bxy = std::fma(c[2], (double)p.y, std::fma(c[1], (double)p.x, c[0]));
+                    const int sa = (int)std::floor(std::fma(c[3], zlo, bxy) - 1.0) + 1;
+                    const int sb = (int)std::floor(std::fma(c[3], zhi, bxy) - 1.0) + 2;
+                    runmax = std::max(runmax, sa);
+                    out.first = std::max(out.first, runmax - sa);
+                    out.second = std::max(out.second, std::min(sb, L - 1) - sa);
+                }
+            }
+        }
+        return out;
+    };
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (int r = 0; r < 2; r++) {
+        std::vector<std::pair<int, int>> part(nth);
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++)
+            th.emplace_back([&, t] { part[t] = pass(r == 1, A * t / nth, A * (t + 1) / nth); });
+        for (auto &x : th) x.join();
+        int jt = 0, sp = 0;
+        for (auto &q : part) {
+            jt = std::max(jt, q.first);
+            sp = std::max(sp, q.second);
+        }
+        jt += 2;  // margins against any rounding difference of the kernel's u
+        sp += 2;
+        int w = 64;
+        while (w <= sp + jt) w *= 2;
+        if (w < L && (*win == 0 || w < *win)) {
+            *win = w;
+            *jit = jt;
+            *span = sp;
+            *rev = r == 1;
+        }
+    }
+}
+
 // p_checked: p_host was already scanned for non-finite values (a request
 // the server validated at submission).
 int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double *xc, const double *yc,
@@ -1049,6 +1125,17 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                 const int at = pos[key[c]]++;
                 xy[at] = make_int2(X, Y);
                 wts[at] = make_float2((float)(1.0 - dvs[c]), (float)dvs[c]);
+            }
+        bool rev = false;
+        col_window(xy, cnt, xc, A, vb, prj->X, &ag.col_win, &ag.col_jit, &ag.col_span, &rev);
+        if (const char *w = std::getenv("MTR_FP_COL_WINDOW"); w && std::atoi(w) == 0) {
+            ag.col_win = 0;
+            rev = false;
+        }
+        if (rev)
+            for (int q = 0; q < nstrip; q++) {
+                std::reverse(xy.begin() + cnt[q], xy.begin() + cnt[q + 1]);
+                std::reverse(wts.begin() + cnt[q], wts.begin() + cnt[q + 1]);
             }
         ALLOC(ag.d_col_xy, sizeof(int2) * xy.size());
         ALLOC(ag.d_col_w, sizeof(float2) * wts.size());
@@ -1359,6 +1446,14 @@ int mtr_agent_tiles(const mtr_ctx *ctx, int agent, int *run, int *lanes_axis)
     const Agent &ag = ctx->agents[agent];
     if (run) *run = ag.col.on ? ag.col.rw : (ag.bpm.sep ? ag.bpm.rw : 0);
     if (lanes_axis) *lanes_axis = ag.col.on ? ag.col.p1y : (ag.bpm.sep ? ag.bpm.P1 : 0);
+    return MTR_OK;
+}
+
+int mtr_agent_fp_window(const mtr_ctx *ctx, int agent, int *slots)
+{
+    MTR_TRY(check_agent(const_cast<mtr_ctx *>(ctx), agent));
+    if (!slots) return fail(MTR_ERR_ARG, "null slots");
+    *slots = ctx->agents[agent].col.on ? ctx->agents[agent].col_win : 0;
     return MTR_OK;
 }
 

@@ -79,6 +79,9 @@ struct AgentDev {
     int nstrip;            // nv + 1 (k0 = -1 .. nv - 1)
     int col_p1y;           // back-projection tiles: lanes along Y (1) or X (0)
     int col_pitch;         // back-projection LDS pitch (pairs per staged row)
+    int col_win;           // FP row window in slots (power of two), 0: the whole width
+    int col_jit;           // FP row window: drop of a column's first slot below an earlier column's
+    int col_span;          // FP row window: a column's last slot past its first
     float *pm, *pn;        // FP partial rows [A][nv][nu]: from strip k and strip k - 1
     float *bpc;            // back-projection normaliser C of every voxel (working layout)
 };

@@ -26,7 +26,7 @@ hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_block
 // and residual (mode 0) or the normaliser into c_out (mode 2).
 int fp_col_blocks(const AgentDev &ag);
 int fp_col_max_nu();
-hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int slots, bool unit,
                          hipStream_t st);
 int fp_col_res_blocks(const AgentDev &ag);
 hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
@@ -37,6 +37,7 @@ hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int block
 // modes 0 and 1).
 int bp_col_blocks(const AgentDev &ag, int rw);
 int bp_col_waves();
+int fp_col_waves();
 int col_elems();
 hipError_t launch_bp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda, int rw,
                          hipStream_t st);

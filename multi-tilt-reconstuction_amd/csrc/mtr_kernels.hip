@@ -1133,6 +1133,52 @@ __device__ __forceinline__ void seg_scan_cls(float &a, float &b, int key)
     }
 }
 
+// A wave's two detector rows (row k0, row k0 + 1) in LDS, interleaved, by
+// slot = detector column + 1.  Either the whole width (msk keeps every
+// slot), or a window of W = msk + 1 slots (a power of two) addressed modulo
+// W: the strip's columns move monotonically along u for every view, and
+// a column's first slot never lies more than jit below the first slot of
+// any column before it, so when the next column's slots (at most span past
+// its first) would leave the window, the slots below its first slot - jit
+// are final: they are written out and cleared.  jit and span come from the
+// host for the agent; W > span + jit.
+struct ColRows {
+    float2 *rw;
+    int msk, L, nu;
+    bool win;
+    int jit, span, lo;
+    float *pm, *pn;  // output rows k0 and k0 + 1 (null: off the detector)
+
+    __device__ __forceinline__ float2 &at(int slot) { return rw[slot & msk]; }
+    // writes out and clears slots [lo, hi)
+    __device__ __forceinline__ void flush(int hi, int lane)
+    {
+        for (int q = lo + lane; q < hi; q += 64) {
+            float2 &c = at(q);
+            const float2 v = c;
+            const int i = q - 1;
+            if (i >= 0 && i < nu) {
+                if (pm) pm[i] = v.x;
+                if (pn) pn[i] = v.y;
+            }
+            c = make_float2(0.0f, 0.0f);
+        }
+        lo = hi > lo ? hi : lo;
+    }
+    // before a column whose planes have u from fma(c3, z, bxy) - 1 over
+    // z in [zs, zs + nz): its first slot from the plane of smallest u
+    __device__ __forceinline__ void column(double c3, double bxy, int zs, int nz, int lane)
+    {
+        if (!win) return;
+        const double zmin = c3 >= 0.0 ? (double)zs : (double)(zs + nz - 1);
+        const int sa = (int)floor(fma(c3, zmin, bxy) - 1.0) + 1;
+        if (sa + span >= lo + msk + 1) {
+            flush(min(sa - jit, L), lane);
+            wave_lds_sync();
+        }
+    }
+};
+
 // One chunk of 64 planes of one column: lane zi - zc holds plane zi with
 // value xv.  Leaders (last lane of each segment) add the segment sums into
 // the wave's interleaved rows rw (row k0, row k0 + 1): detector column
@@ -1141,7 +1187,7 @@ __device__ __forceinline__ void seg_scan_cls(float &a, float &b, int key)
 // touches these rows, so the summation order is fixed.
 template <int CLS, bool TAIL>
 __device__ __forceinline__ void col_chunk(float xv, int zi, int nz, double c3, double bxy, int zs, float wm,
-                                          float wn, float2 *rw, int nu, int lane)
+                                          float wn, ColRows &rw, int nu, int lane)
 {
     const bool ok = !TAIL || zi < nz;
     // the reference's chain: fma(xc3, Z, fma(xc2, Y, fma(xc1, X, xc0))) - 1
@@ -1157,17 +1203,17 @@ __device__ __forceinline__ void col_chunk(float xv, int zi, int nz, double c3, d
     const bool lead = ok && nkey != key;
     const int bi = key - kKeyBias + 1;  // slot of detector column floor(u) (column -1 -> slot 0)
     if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi];
+        float2 q = rw.at(bi);
         q.x = fmaf(wm, ea, q.x);
         q.y = fmaf(wn, ea, q.y);
-        rw[bi] = q;
+        rw.at(bi) = q;
     }
     wave_lds_sync();
     if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi + 1];
+        float2 q = rw.at(bi + 1);
         q.x = fmaf(wm, eb, q.x);
         q.y = fmaf(wn, eb, q.y);
-        rw[bi + 1] = q;
+        rw.at(bi + 1) = q;
     }
     wave_lds_sync();
 }
@@ -1178,7 +1224,7 @@ __device__ __forceinline__ void col_chunk(float xv, int zi, int nz, double c3, d
 // so the wave waits on global memory once per column at most.
 template <class T, bool UNIT, int CLS, int G>
 __device__ __forceinline__ void col_strip(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
-                                          double c3, float2 *rw, int lane)
+                                          double c3, ColRows &rw, int lane)
 {
     const auto x = gview(reinterpret_cast<const T *>(ag.x));
     const int nz = ag.Z, zs = ag.Zs, nu = ag.nu;
@@ -1198,6 +1244,7 @@ __device__ __forceinline__ void col_strip(const AgentDev &ag, int j0, int j1, do
         const int2 xy = ag.col_xy[j];
         const float2 cw = ag.col_w[j];
         const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
+        rw.column(c3, bxy, zs, nz, lane);
         const auto xcol = column(j);
         int zc = 0;
         if constexpr (G > 0) {
@@ -1239,7 +1286,7 @@ __device__ __forceinline__ void col_strip(const AgentDev &ag, int j0, int j1, do
 // about R times shorter than over single planes.
 template <class T, bool UNIT, int R, int CLS>
 __device__ __forceinline__ void col_chunk_r(const float (&xv)[R], int zc, double c3, double bxy, int zs, float wm,
-                                            float wn, float2 *rw, int nu, int lane, int dt)
+                                            float wn, ColRows &rw, int nu, int lane, int dt)
 {
     float A = 0.0f, B = 0.0f, At = 0.0f, Bt = 0.0f;
     int f0 = 0;
@@ -1273,29 +1320,29 @@ __device__ __forceinline__ void col_chunk_r(const float (&xv)[R], int zc, double
     const int bi = f0 + 1;  // slot of detector column f0
     const bool own = tail && lane == 63;
     if (lead && (unsigned)bi <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi];
+        float2 q = rw.at(bi);
         q.x = fmaf(wm, A, q.x);
         q.y = fmaf(wn, A, q.y);
-        rw[bi] = q;
+        rw.at(bi) = q;
     }
     if (own && (unsigned)(bi + dt) <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi + dt];
+        float2 q = rw.at(bi + dt);
         q.x = fmaf(wm, At, q.x);
         q.y = fmaf(wn, At, q.y);
-        rw[bi + dt] = q;
+        rw.at(bi + dt) = q;
     }
     wave_lds_sync();
     if (lead && (unsigned)(bi + 1) <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi + 1];
+        float2 q = rw.at(bi + 1);
         q.x = fmaf(wm, B, q.x);
         q.y = fmaf(wn, B, q.y);
-        rw[bi + 1] = q;
+        rw.at(bi + 1) = q;
     }
     if (own && (unsigned)(bi + 1 + dt) <= (unsigned)(nu + 1)) {
-        float2 q = rw[bi + 1 + dt];
+        float2 q = rw.at(bi + 1 + dt);
         q.x = fmaf(wm, Bt, q.x);
         q.y = fmaf(wn, Bt, q.y);
-        rw[bi + 1 + dt] = q;
+        rw.at(bi + 1 + dt) = q;
     }
     wave_lds_sync();
 }
@@ -1306,7 +1353,7 @@ __device__ __forceinline__ void col_chunk_r(const float (&xv)[R], int zc, double
 // go through col_chunk with the Hillis-Steele scan.
 template <class T, bool UNIT, int R, int G, int CLS>
 __device__ __forceinline__ void col_strip_r(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
-                                            double c3, float2 *rw, int lane)
+                                            double c3, ColRows &rw, int lane)
 {
     const auto x = gview(reinterpret_cast<const T *>(ag.x));
     const int nz = ag.Z, zs = ag.Zs, nu = ag.nu, dt = c3 > 0.0 ? 1 : -1;
@@ -1328,6 +1375,7 @@ __device__ __forceinline__ void col_strip_r(const AgentDev &ag, int j0, int j1, 
         const int2 xy = ag.col_xy[j];
         const float2 cw = ag.col_w[j];
         const double bxy = fma(c2, (double)xy.y, fma(c1, (double)xy.x, c0));
+        rw.column(c3, bxy, zs, nz, lane);
         const auto xcol = column(j);
         float xg[G][R];
 #pragma unroll
@@ -1374,7 +1422,7 @@ __device__ __forceinline__ void col_strip_r(const AgentDev &ag, int j0, int j1, 
 // longest run of lanes sharing a first key, ceil(maxrun / R).
 template <class T, bool UNIT, int R, int G>
 __device__ __forceinline__ void col_strip_rc(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
-                                             double c3, float2 *rw, int lane, int maxrun)
+                                             double c3, ColRows &rw, int lane, int maxrun)
 {
     const int m = (maxrun + R - 1) / R;
     if (m <= 2) col_strip_r<T, UNIT, R, G, 1>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
@@ -1388,7 +1436,7 @@ __device__ __forceinline__ void col_strip_rc(const AgentDev &ag, int j0, int j1,
 // below 64 planes: partial chunks only).
 template <class T, bool UNIT, int CLS>
 __device__ __forceinline__ void col_strip_g(const AgentDev &ag, int j0, int j1, double c0, double c1, double c2,
-                                            double c3, float2 *rw, int lane)
+                                            double c3, ColRows &rw, int lane)
 {
     if (ag.Z >= 256) col_strip<T, UNIT, CLS, 4>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     else if (ag.Z >= 128) col_strip<T, UNIT, CLS, 2>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
@@ -1429,9 +1477,23 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     const int a = (idx - sx * nvg) * kColWaves + wid;
     if (a >= ag.A) return;  // no block-wide barrier below
     const int nu = ag.nu, nv = ag.nv, L = nu + 2;
-    // the wave's two detector rows, interleaved: (row k0, row k0 + 1)
-    float2 *rw = reinterpret_cast<float2 *>(col_rows) + wid * L;
-    for (int q = lane; q < L; q += 64) rw[q] = make_float2(0.0f, 0.0f);
+    const int k0 = s - 1;
+    const int64_t row = ((int64_t)a * nv + k0) * nu;
+    // the wave's two detector rows, interleaved: (row k0, row k0 + 1), the
+    // whole width or a window of col_win slots
+    const int nslot = ag.col_win > 0 ? ag.col_win : L;
+    ColRows rw;
+    rw.rw = reinterpret_cast<float2 *>(col_rows) + wid * nslot;
+    rw.win = ag.col_win > 0;
+    rw.msk = rw.win ? ag.col_win - 1 : 0x7fffffff;
+    rw.L = L;
+    rw.nu = nu;
+    rw.jit = ag.col_jit;
+    rw.span = ag.col_span;
+    rw.lo = 0;
+    rw.pm = k0 >= 0 ? ag.pm + row : nullptr;
+    rw.pn = k0 + 1 < nv ? ag.pn + row + nu : nullptr;
+    for (int q = lane; q < nslot; q += 64) rw.rw[q] = make_float2(0.0f, 0.0f);
     wave_lds_sync();
     const ViewCoef vc = ag.coef[a];
     const double c0 = vc.c[0], c1 = vc.c[1], c2 = vc.c[2], c3 = vc.c[3];
@@ -1454,13 +1516,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     else if (maxrun <= 8) col_strip_g<T, UNIT, 8>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     else col_strip_g<T, UNIT, 64>(ag, j0, j1, c0, c1, c2, c3, rw, lane);
     wave_lds_sync();
-    const int k0 = s - 1;
-    const int64_t row = ((int64_t)a * nv + k0) * nu;
-    for (int i = lane; i < nu; i += 64) {
-        const float2 q = rw[i + 1];
-        if (k0 >= 0) ag.pm[row + i] = q.x;
-        if (k0 + 1 < nv) ag.pn[row + nu + i] = q.y;
-    }
+    rw.flush(L, lane);  // the rest of the rows (all of them without a window)
 }
 
 // s = pm + pn; MODE 0: residual r = p - (c != 0 ? s/c : s) with FP64
@@ -2406,22 +2462,22 @@ int fp_col_blocks(const AgentDev &ag)
 int fp_col_max_nu() { return (int)(65536 / (kColWaves * 2 * sizeof(float))) - 2; }
 
 template <class T>
-static hipError_t launch_fp_col_t(const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+static hipError_t launch_fp_col_t(const AgentDev *d_ags, int nag, int max_blocks, int slots, bool unit,
                                   hipStream_t st)
 {
-    const size_t shm = (size_t)kColWaves * 2 * (nu_max + 2) * sizeof(float);
+    const size_t shm = (size_t)kColWaves * slots * sizeof(float2);
     dim3 grid(max_blocks, nag), block(64 * kColWaves);
     if (unit) hipLaunchKernelGGL((k_fp_col<T, true>), grid, block, shm, st, d_ags);
     else hipLaunchKernelGGL((k_fp_col<T, false>), grid, block, shm, st, d_ags);
     return hipGetLastError();
 }
 
-hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nu_max, bool unit,
+hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int slots, bool unit,
                          hipStream_t st)
 {
-    if (prec == MTR_PREC_BF16) return launch_fp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, nu_max, unit, st);
-    if (prec == MTR_PREC_FP16) return launch_fp_col_t<__half>(d_ags, nag, max_blocks, nu_max, unit, st);
-    return launch_fp_col_t<float>(d_ags, nag, max_blocks, nu_max, unit, st);
+    if (prec == MTR_PREC_BF16) return launch_fp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, slots, unit, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_col_t<__half>(d_ags, nag, max_blocks, slots, unit, st);
+    return launch_fp_col_t<float>(d_ags, nag, max_blocks, slots, unit, st);
 }
 
 int fp_col_res_blocks(const AgentDev &ag)
@@ -2456,6 +2512,7 @@ int bp_col_blocks(const AgentDev &ag, int rw)
 }
 
 int bp_col_waves() { return kColBpWaves; }
+int fp_col_waves() { return kColWaves; }
 
 template <class T, int RW>
 static void launch_bp_col_rw(dim3 grid, const AgentDev *d_ags, int mode, float lambda, hipStream_t st)

@@ -449,3 +449,32 @@ def test_stream_fp_columns_per_wave_give_identical_sums(mtr, orc, name, kw):
     prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
     ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 2, 0.5)
     assert orc.rel_l2(outs[0][0], ref) < SOLVE_TOL
+
+
+def test_column_fp_row_window_on_a_wide_detector(mtr, orc):
+    # a 1000-column detector: whole rows would leave fewer than six blocks
+    # per CU, so each wave keeps a window of its rows that the strip walk
+    # writes out as it moves; against the oracle, and against whole rows
+    pb = problem(mtr, [45.0], X=600, Y=600, Z=48, views=7, seed=27, nu=1000, nv=1000)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    say("column FP window")
+    outs = []
+    old = os.environ.get("MTR_FP_COL_WINDOW")
+    try:
+        for w in ("1", "0"):
+            os.environ["MTR_FP_COL_WINDOW"] = w
+            with load_ctx(mtr, pb) as c:
+                assert c.agent_kernels(0)[0] == mtr.FPK_COLUMN
+                win = c.agent_fp_window(0)
+                assert (win > 0) == (w == "1") and win < prj.X + 2
+                c.bpt(0, out=False)
+                outs.append(c.sirt(0, 2, 0.5))
+    finally:
+        if old is None:
+            os.environ.pop("MTR_FP_COL_WINDOW", None)
+        else:
+            os.environ["MTR_FP_COL_WINDOW"] = old
+    ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 2, 0.5)
+    assert orc.rel_l2(outs[0], ref) < SOLVE_TOL
+    assert orc.rel_l2(outs[1], ref) < SOLVE_TOL
+    assert orc.rel_l2(outs[0], outs[1]) < 1e-6
