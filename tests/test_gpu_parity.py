@@ -478,3 +478,34 @@ def test_column_fp_row_window_on_a_wide_detector(mtr, orc):
     assert orc.rel_l2(outs[0], ref) < SOLVE_TOL
     assert orc.rel_l2(outs[1], ref) < SOLVE_TOL
     assert orc.rel_l2(outs[0], outs[1]) < 1e-6
+
+
+TINY = [
+    ("tiny_slab0", dict(phis=[0.0], X=5, Y=3, Z=2, nu=4, nv=3, views=3)),
+    ("tiny_slab90", dict(phis=[90.0], X=3, Y=5, Z=2, nu=4, nv=3, views=3)),
+    ("tiny_axisZ", dict(phis=[0.0], kind=1, X=3, Y=4, Z=2, nu=3, nv=3, views=3)),
+    ("tiny_col45", dict(phis=[45.0], X=4, Y=5, Z=3, nu=6, nv=6, views=3)),
+    ("one_voxel", dict(phis=[45.0], X=1, Y=1, Z=1, nu=2, nv=2, views=2)),
+    ("one_plane", dict(phis=[0.0], X=7, Y=6, Z=1, nu=8, nv=7, views=5)),
+]
+
+
+@pytest.mark.parametrize("name,kw", TINY, ids=[g[0] for g in TINY])
+def test_tiny_problems_match_oracle(mtr, orc, name, kw):
+    # a handful of voxels and detector pixels: every tile, strip and chunk is
+    # partial, most views' footprints are cut by the detector's edges
+    pb = problem(mtr, **kw)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    say("tiny", name)
+    with load_ctx(mtr, pb) as c:
+        say("kernels", c.agent_kernels(0))
+        xb = c.bpt(0)
+        x_ref = orc.bpt(prj, vol, xc, yc, p)
+        assert orc.rel_l2(xb, x_ref) < OP_TOL
+        x2 = c.sirt(0, 2, 0.7)
+        assert orc.rel_l2(x2, orc.sirt(prj, vol, xc, yc, p, x_ref, 2, 0.7)) < SOLVE_TOL
+    pb2 = problem(mtr, **dict(kw, phis=[kw["phis"][0], kw["phis"][0] + 90.0]))
+    with load_ctx(mtr, pb2) as c:
+        out = c.mace_solve(2, 2, 0.5, 0.5)
+    ref = orc.mace(pb2.prjs, pb2.vol, pb2.xcs, pb2.ycs, pb2.ps, 2, 2, 0.5, 0.5)
+    assert orc.rel_l2(out, ref) < SOLVE_TOL
