@@ -990,8 +990,14 @@ void col_window(const std::vector<int2> &xy, const std::vector<int> &cnt, const 
     for (int r = 0; r < 2; r++) {
         std::vector<std::pair<int, int>> part(nth);
         std::vector<std::thread> th;
-        for (int t = 0; t < nth; t++)
-            th.emplace_back([&, t] { part[t] = pass(r == 1, A * t / nth, A * (t + 1) / nth); });
+        for (int t = 0; t < nth; t++) {
+            auto job = [&, t] { part[t] = pass(r == 1, A * t / nth, A * (t + 1) / nth); };
+            try {
+                th.emplace_back(job);
+            } catch (...) {
+                job();  // no thread to spare: this share of the views here
+            }
+        }
         for (auto &x : th) x.join();
         int jt = 0, sp = 0;
         for (auto &q : part) {
