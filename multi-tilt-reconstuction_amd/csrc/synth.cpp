@@ -84,10 +84,16 @@ void parallel_for(int n, int threads, F f)
     }
     std::vector<std::thread> pool;
     pool.reserve(threads);
-    for (int t = 0; t < threads; t++)
-        pool.emplace_back([=]() {
+    for (int t = 0; t < threads; t++) {
+        auto job = [=]() {
             for (int i = t; i < n; i += threads) f(i);
-        });
+        };
+        try {
+            pool.emplace_back(job);
+        } catch (...) {
+            job();  // no thread to spare: this share here
+        }
+    }
     for (auto &th : pool) th.join();
 }
 
