@@ -228,7 +228,7 @@ int fp_stream_nc(const std::vector<FpPlan> &plan, const double *xc, int *W_out)
 struct ColMode {
     bool on = false;
     int p1y = 0;    // lanes along Y
-    int pitch = 0;  // staged pairs per row (odd)
+    int pitch = 0;  // staged pairs per row (8 mod 16 where it fits, else odd)
     int rw = 32;
 };
 
@@ -253,8 +253,16 @@ ColMode col_mode(const double *xc, const double *yc, int A, const mtr_projection
                              (rw - 1) * std::fabs(xc[4 * v + 3]);
             wmax = std::max(wmax, w);
         }
+        // pitch = 8 (mod 16) pairs: a lane one row further reads 8 bank
+        // pairs away, so the lanes of a wave, spread over rows and columns,
+        // meet on fewer LDS banks (3% faster than an odd pitch at 45 and 60
+        // deg; 1 (mod 16) was 2.3x slower at 45); odd where that does not fit
         int pitch = (int)std::floor(wmax) + 4;
-        if (pitch % 2 == 0) pitch++;
+        while (pitch % 16 != 8) pitch++;
+        if (rows * pitch > col_elems()) {
+            pitch = (int)std::floor(wmax) + 4;
+            if (pitch % 2 == 0) pitch++;
+        }
         const int elems = rows * pitch;
         if (elems <= col_elems() && (best < 0 || elems < best)) {
             best = elems;
