@@ -480,6 +480,44 @@ def test_column_fp_row_window_on_a_wide_detector(mtr, orc):
     assert orc.rel_l2(outs[0], outs[1]) < 1e-6
 
 
+# deep columns on a 1000-column detector: the row window together with
+# lanes owning 4 and 2 planes, grouped chunk reads and the partial last
+# chunk, for in-plane axes whose u rises and falls along the memory order
+# (the strip walk runs backwards for the latter)
+WINDOW_DEEP = [
+    ("col30_z300", dict(phis=[30.0], X=40, Y=36, Z=300, views=13)),
+    ("col135_z400", dict(phis=[135.0], X=20, Y=16, Z=400, views=13)),
+    ("col60_z520", dict(phis=[60.0], X=16, Y=20, Z=520, views=13)),
+]
+
+
+@pytest.mark.parametrize("name,kw", WINDOW_DEEP, ids=[g[0] for g in WINDOW_DEEP])
+def test_column_fp_row_window_on_deep_columns(mtr, orc, name, kw):
+    pb = problem(mtr, seed=29, nu=1000, **kw)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    say("column FP window, deep", name)
+    outs = []
+    old = os.environ.get("MTR_FP_COL_WINDOW")
+    try:
+        for w in ("1", "0"):
+            os.environ["MTR_FP_COL_WINDOW"] = w
+            with load_ctx(mtr, pb) as c:
+                assert c.agent_kernels(0)[0] == mtr.FPK_COLUMN
+                win = c.agent_fp_window(0)
+                assert (win > 0) == (w == "1") and win < prj.X + 2
+                c.bpt(0, out=False)
+                outs.append(c.sirt(0, 2, 0.5))
+    finally:
+        if old is None:
+            os.environ.pop("MTR_FP_COL_WINDOW", None)
+        else:
+            os.environ["MTR_FP_COL_WINDOW"] = old
+    ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 2, 0.5)
+    assert orc.rel_l2(outs[0], ref) < SOLVE_TOL
+    assert orc.rel_l2(outs[1], ref) < SOLVE_TOL
+    assert orc.rel_l2(outs[0], outs[1]) < 1e-6
+
+
 TINY = [
     ("tiny_slab0", dict(phis=[0.0], X=5, Y=3, Z=2, nu=4, nv=3, views=3)),
     ("tiny_slab90", dict(phis=[90.0], X=3, Y=5, Z=2, nu=4, nv=3, views=3)),
