@@ -279,8 +279,9 @@ int mtr_sync(mtr_ctx *ctx);
  * request is a volume box and K tilt series (each with its own detector
  * size and geometry) plus its solve parameters (M MACE steps of S SIRT
  * iterations, lambda, rho); the server copies it on submission.  Requests
- * are admitted in arrival order while their HBM footprint fits the budget
- * and fewer than max_active are active; admission uploads the series and
+ * are admitted (in arrival order unless mtr_server_set_admission says
+ * otherwise) while their HBM footprint fits the budget and fewer than
+ * max_active are active; admission uploads the series and
  * forms the BPT-sum initial model.  Each mtr_server_step runs one MACE
  * step of every active request, the SIRT iterations of all their agents
  * batched into shared launches, and retires the requests that finished
@@ -290,6 +291,15 @@ int mtr_sync(mtr_ctx *ctx);
 typedef struct mtr_server mtr_server;
 int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out);
 int mtr_server_destroy(mtr_server *srv);
+/* Admission order.  MTR_ADMIT_ARRIVAL (the default): oldest request first.
+ * MTR_ADMIT_SMALLEST: the pending request with the least work (voxel-view
+ * updates of its whole solve, (2SM + 1) N sum_a A_a) first, except that a
+ * request that has waited longer than max_wait_s seconds goes first, in
+ * arrival order; this shortens the median latency under load and bounds
+ * what it costs the largest requests.  The chosen request waits for HBM
+ * rather than be overtaken.  Results do not depend on the order. */
+enum { MTR_ADMIT_ARRIVAL = 0, MTR_ADMIT_SMALLEST = 1 };
+int mtr_server_set_admission(mtr_server *srv, int policy, double max_wait_s);
 /* Device bytes a request of this shape holds while active (upper bound). */
 int mtr_server_footprint(const mtr_server *srv, const mtr_volume_size *vol, int K,
                          const mtr_projection_size *prj, size_t *bytes);

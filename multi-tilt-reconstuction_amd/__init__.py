@@ -37,6 +37,7 @@ from ._lib import (  # noqa: F401
 )
 
 PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
+ADMIT_ARRIVAL, ADMIT_SMALLEST = 0, 1
 FPK_GENERAL, FPK_STREAM, FPK_COLUMN = 0, 1, 2
 BPK_GENERAL, BPK_SEPARABLE, BPK_COLUMN = 0, 1, 2
 FORM_FAST, FORM_PLAIN = 0, 1
@@ -381,6 +382,12 @@ class Server:
 
     def submit_problem(self, pb) -> int:
         return self.submit(pb.vol, pb.prjs, pb.xcs, pb.ycs, pb.ps, pb.M, pb.S, pb.lam, pb.rho)
+
+    def set_admission(self, policy: int, max_wait: float = 0.0) -> None:
+        """ADMIT_ARRIVAL (oldest first) or ADMIT_SMALLEST (least work first,
+        except that a request waiting longer than max_wait seconds goes
+        first, in arrival order)."""
+        _check(lib.mtr_server_set_admission(self._h, int(policy), _ct.c_double(max_wait)), "server_set_admission")
 
     def step(self) -> int:
         n = _ct.c_int()

@@ -2197,6 +2197,7 @@ struct ServerReq {
     int step = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t bytes = 0;
+    double work = 0;  // voxel-view updates of the whole solve: (2SM + 1) N sum_a A_a
     mtr_ctx *ctx = nullptr;
     std::vector<float> result;
     std::string error;
@@ -2210,6 +2211,8 @@ struct mtr_server {
     int precision = MTR_PREC_FP32;
     size_t budget = 0;
     int max_active = 0;
+    int admit_policy = MTR_ADMIT_ARRIVAL;
+    double max_wait = 0;  // seconds, MTR_ADMIT_SMALLEST only
     mtr_ctx *eng = nullptr;  // engine context: the stream every request shares
     size_t used = 0;
     int64_t next_id = 1;
@@ -2355,6 +2358,11 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
     r->lambda = lambda;
     r->rho = rho;
     r->bytes = bytes;
+    {
+        double views = 0;
+        for (int a = 0; a < K; a++) views += prj[a].AngN;
+        r->work = (2.0 * S * M + 1.0) * (double)vol->X * vol->Y * vol->Z * views;
+    }
     r->t_submit = srv->now();
     if (request_id) *request_id = r->id;
     srv->pending.push_back(r.get());
@@ -2362,10 +2370,23 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
     return MTR_OK;
 }
 
-// One scheduling tick: admit pending requests in arrival order while the
-// HBM budget and the slot count allow, run one MACE step of every active
-// request with their SIRT iterations batched into shared launches, then
-// retire the requests that finished.
+int mtr_server_set_admission(mtr_server *srv, int policy, double max_wait_s)
+{
+    if (!srv) return fail(MTR_ERR_ARG, "null server");
+    if (policy != MTR_ADMIT_ARRIVAL && policy != MTR_ADMIT_SMALLEST)
+        return fail(MTR_ERR_ARG, "unknown admission policy %d", policy);
+    if (!(max_wait_s >= 0) || !std::isfinite(max_wait_s))
+        return fail(MTR_ERR_ARG, "maximum wait must be finite and non-negative");
+    srv->admit_policy = policy;
+    srv->max_wait = max_wait_s;
+    return MTR_OK;
+}
+
+// One scheduling tick: admit pending requests while the HBM budget and the
+// slot count allow (in arrival order, or smallest solve first with a bound
+// on the wait), run one MACE step of every active request with their SIRT
+// iterations batched into shared launches, then retire the requests that
+// finished.
 int mtr_server_step(mtr_server *srv, int *completed)
 {
     if (!srv) return fail(MTR_ERR_ARG, "null server");
@@ -2373,9 +2394,17 @@ int mtr_server_step(mtr_server *srv, int *completed)
     MTR_TRY(set_device(srv->eng));
     int done = 0;
     while (!srv->pending.empty() && (int)srv->active.size() < srv->max_active) {
-        ServerReq *r = srv->pending.front();
+        // arrival order: the oldest request; smallest first: the oldest one
+        // that has waited longer than max_wait, else the least work (the
+        // earlier arrival on a tie).  The pick waits for HBM rather than be
+        // overtaken, so a large request is never starved.
+        auto pick = srv->pending.begin();
+        if (srv->admit_policy == MTR_ADMIT_SMALLEST && srv->now() - (*pick)->t_submit <= srv->max_wait)
+            for (auto it = srv->pending.begin(); it != srv->pending.end(); ++it)
+                if ((*it)->work < (*pick)->work) pick = it;
+        ServerReq *r = *pick;
         if (srv->used + r->bytes > srv->budget) break;
-        srv->pending.pop_front();
+        srv->pending.erase(pick);
         srv->used += r->bytes;
         r->t_admit = srv->now();
         const int rc = admit(srv, r);

@@ -6,8 +6,9 @@ and iteration counts that arrive while it is active, and by a standalone
 context.  Batching only puts several agents into one launch; each agent's
 blocks, sums and reductions are its own, so the three results are compared
 bit for bit (a stricter bar than the FP32 tolerance, 1e-5 relative L2,
-which the oracle comparison uses).  Admission follows arrival order within
-the HBM budget and the slot count.
+which the oracle comparison uses).  Admission follows arrival order (or
+smallest work first, with a bound on the wait) within the HBM budget and
+the slot count.
 """
 import sys
 
@@ -94,6 +95,37 @@ def test_admission_follows_hbm_budget_in_arrival_order(mtr):
     # admitted in arrival order, each after its predecessor retired
     for q in range(1, 3):
         assert t[q][1] >= t[q - 1][2]
+
+
+def test_smallest_first_admission_order_and_results(mtr):
+    """With one slot, smallest-first admission runs the queued requests in
+    order of work (voxel-view updates of the whole solve) instead of
+    arrival, a request that has waited past the bound goes first, and every
+    result equals the one from arrival-order admission bit for bit."""
+    pbs = [req(mtr, 40, 40, 12, [0.0, 90.0], 70, M=2),  # largest
+           req(mtr, 24, 24, 8, [0.0], 71, M=2),  # smallest
+           req(mtr, 32, 32, 10, [0.0, 90.0], 72, M=2)]
+    res = {}
+    for policy, wait in ((mtr.ADMIT_ARRIVAL, 0.0), (mtr.ADMIT_SMALLEST, 1e6), (mtr.ADMIT_SMALLEST, 0.0)):
+        say("admission policy", policy, "max wait", wait)
+        with mtr.Server(0, max_active=1) as srv:
+            srv.set_admission(policy, wait)
+            ids = [srv.submit_problem(p) for p in pbs]
+            drain(srv, 3)
+            t = [srv.times(r)[1] for r in ids]
+            res[policy, wait] = [srv.result(r)[0] for r in ids]
+        order = sorted(range(3), key=lambda q: t[q])
+        if policy == mtr.ADMIT_SMALLEST and wait > 0:
+            assert order == [1, 2, 0]
+        else:  # arrival order, or every request past a zero wait bound
+            assert order == [0, 1, 2]
+    for key, got in res.items():
+        for q in range(3):
+            assert np.array_equal(got[q], res[mtr.ADMIT_ARRIVAL, 0.0][q]), (key, q)
+    with mtr.Server(0) as srv:
+        for bad in ((2, 0.0), (mtr.ADMIT_SMALLEST, -1.0), (mtr.ADMIT_SMALLEST, float("nan"))):
+            with pytest.raises(mtr.MtrError):
+                srv.set_admission(*bad)
 
 
 def test_slot_limit_and_oversized_request(mtr):
