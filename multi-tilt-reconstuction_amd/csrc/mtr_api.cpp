@@ -391,6 +391,8 @@ struct mtr_ctx {
     float *d_s = nullptr, *d_cc = nullptr;
     int64_t sc_cap = 0;
     bool autotune = true;
+    bool joint_tune = true;  // the joint run-length stage after the initial model
+    std::string joint_key;   // agents it last ran for
     // graph of one fast MACE step, replayed for every step of a solve
     bool use_graphs = true;
     hipGraphExec_t step_graph = nullptr;
@@ -1639,6 +1641,98 @@ static int init_from_bpt(mtr_ctx *ctx)
     return MTR_OK;
 }
 
+// Joint choice of the separable back-projections' run lengths.  In the
+// whole-context solve each agent's back-projection follows the forward
+// projection of every agent and the back-projections sorted before it, and
+// the cache state this leaves can favour another run length than the one an
+// agent is fastest with alone (at 512x512x128 with axes at 0 and 90 deg:
+// runs of 16 for both agents, against 32 chosen one agent at a time).  Run
+// after the initial model: it times whole-context SIRT iterations with
+// lambda = 0, whose update adds fl32(S/C) * 0 = 0 to every voxel, so the
+// working volumes keep their values; the residuals are rewritten by the
+// solve's first forward projection.  Candidates are the per-agent choice and
+// every separable agent at one common run length, timed in turns (only the
+// separable back-projections, each after a whole-context forward
+// projection), and the fastest is kept and cached under a hash of the keys
+// of all the context's agents.
+static int tune_joint(mtr_ctx *ctx)
+{
+    const int K = (int)ctx->agents.size();
+    std::vector<int> sep;
+    for (int a = 0; a < K; a++)
+        if (!ctx->agents[a].col.on && ctx->agents[a].bpm.sep) sep.push_back(a);
+    if (!ctx->autotune || !ctx->joint_tune || sep.empty()) return MTR_OK;
+    std::string all;
+    for (const auto &ag : ctx->agents) all += tune_key(ctx, ag) + "|";
+    char key[64];
+    std::snprintf(key, sizeof key, "joint K%d %016llx", K, (unsigned long long)fnv(all.data(), all.size()));
+    if (ctx->joint_key == key) return MTR_OK;
+    std::vector<int> own;
+    for (int a : sep) own.push_back(ctx->agents[a].bpm.rw);
+    auto apply = [&](int run) -> int {
+        for (size_t q = 0; q < sep.size(); q++) MTR_TRY(apply_tiles(ctx, sep[q], run ? run : own[q], 0));
+        return MTR_OK;
+    };
+    TuneCache &tc = tune_cache();
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        tune_cache_load(tc);
+        auto it = tc.kv.find(key);
+        if (it != tc.kv.end()) {
+            ctx->joint_key = key;
+            return apply(it->second.first);
+        }
+    }
+    std::vector<int> cand = {0};  // 0: the per-agent choice
+    for (int run : {16, 32}) {
+        bool same = true;
+        for (int r : own) same = same && r == run;
+        if (!same) cand.push_back(run);
+    }
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::vector<float> best(cand.size(), 1e30f);
+    int rc = MTR_OK;
+    for (int rep = 0; rep < 4 && rc == MTR_OK; rep++)
+        for (size_t c = 0; c < cand.size() && rc == MTR_OK; c++) {
+            if ((rc = apply(cand[c])) != MTR_OK) break;
+            // the separable agents are contiguous in the kind-sorted table
+            int s0 = K, s1 = 0;
+            for (int i = 0; i < K; i++) {
+                const Agent &ag = ctx->agents[ctx->bp_order[i]];
+                if (!ag.col.on && ag.bpm.sep) {
+                    s0 = std::min(s0, i);
+                    s1 = i + 1;
+                }
+            }
+            const Batch F = ctx_sorted(ctx, ctx->fp_order, ctx->d_ags_fp);
+            const Batch Bk = ctx_sorted(ctx, ctx->bp_order, ctx->d_ags_bp);
+            float t = 0.f;
+            if (fp_batch(ctx, F, 0, K) != MTR_OK || bp_batch(ctx, Bk, 0, s0, false, 0.0f) != MTR_OK ||
+                hipEventRecord(e0, ctx->stream) != hipSuccess || bp_batch(ctx, Bk, s0, s1 - s0, false, 0.0f) != MTR_OK ||
+                hipEventRecord(e1, ctx->stream) != hipSuccess || bp_batch(ctx, Bk, s1, K - s1, false, 0.0f) != MTR_OK ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                rc = fail(MTR_ERR_HIP, "joint autotuner timing failed");
+            best[c] = std::min(best[c], t);
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->timing = timing;
+    if (rc != MTR_OK) return rc;
+    size_t pick = 0;
+    for (size_t c = 1; c < cand.size(); c++)
+        if (best[c] < best[pick]) pick = c;
+    {
+        std::lock_guard<std::mutex> lk(tc.mu);
+        tune_cache_store(tc, key, cand[pick], 0);
+    }
+    ctx->joint_key = key;
+    return apply(cand[pick]);
+}
+
 int mtr_mace_init(mtr_ctx *ctx)
 {
     if (!ctx) return fail(MTR_ERR_ARG, "null context");
@@ -1648,7 +1742,8 @@ int mtr_mace_init(mtr_ctx *ctx)
     MTR_TRY(tune_all(ctx));
     // BPT of every agent into its working volume, batched per kernel kind.
     MTR_TRY(bp_group(ctx, 0, (int)ctx->agents.size(), true, 1.0f));
-    return init_from_bpt(ctx);
+    MTR_TRY(init_from_bpt(ctx));
+    return tune_joint(ctx);
 }
 
 // One fast MACE step (MultiAxis.cpp:40-71): S SIRT iterations of every
@@ -2262,6 +2357,7 @@ int admit(mtr_server *srv, ServerReq *r)
     c->stream = srv->eng->stream;
     c->own_stream = false;
     c->precision = srv->precision;
+    c->joint_tune = false;  // its agents share launches with other requests' agents
     r->ctx = c;
     MTR_TRY(alloc_tables(c));
     MTR_TRY(mtr_ctx_set_volume(c, &r->vol));
