@@ -327,8 +327,10 @@ def main() -> int:
             f"{serve['throughput_req_per_s']:.2f} req/s")
 
     value = steps / wall
+    axes = {1: "single-axis", 2: "dual-axis", 3: "three-axis", 4: "four-axis"}.get(pb.K, f"{pb.K}-axis")
     out = {
-        "metric": "MACE reconstructions/sec (dual-axis 512x512x128, 2x61 views, 10 MACE x 5 SIRT)",
+        "metric": f"MACE reconstructions/sec ({axes} {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, {pb.K}x{A} views, "
+                  f"{pb.M} MACE x {pb.S} SIRT)",
         "value": value,
         "unit": "reconstructions/s",
         "n_gpus": 1,
