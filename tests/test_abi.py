@@ -176,3 +176,14 @@ def test_server_argument_checks(mtr):
     assert mtr.lib.mtr_server_create(0, 7, 1 << 30, 4, ct.byref(out)) == -1  # unknown precision
     assert mtr.lib.mtr_server_create(0, 0, 0, 4, ct.byref(out)) == -1        # empty budget
     assert mtr.lib.mtr_server_create(0, 0, 1 << 30, 0, ct.byref(out)) == -1  # no slots
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # one GPU per box: the bench says so and exits before touching a device
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
