@@ -187,7 +187,7 @@ def main() -> int:
                     help="concurrently active requests (8: lowest p99 on the seeded stream, 2-64 swept)")
     ap.add_argument("--admit", default="smallest", choices=["arrival", "smallest"],
                     help="serving admission order: arrival, or least work first with a bound on the wait "
-                         "(on the seeded stream p50 9.1 s -> 4.6-7.0 s, p99 about the same)")
+                         "(on the seeded stream p50 about 3.7 s lower at the same p99)")
     ap.add_argument("--admit-max-wait", type=float, default=6.0,
                     help="seconds a request may wait before it is admitted in arrival order (--admit smallest)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=256,
