@@ -52,6 +52,37 @@ def test_struct_layouts(mtr):
     assert ct.sizeof(mtr.Slice) == 2 * 4
 
 
+def _header_fields(struct):
+    txt = open(os.path.join(ROOT, "include", "mtr.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + struct + r"\s*;", txt).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if decl:
+            names += [n.strip() for n in decl.split(None, 1)[1].split(",")]
+    return names
+
+
+def _stub_fields(cls):
+    """Field names of a ctypes class in the INTEGRATION.md stub."""
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    stub = re.search(r"```python\n(.*?)```", txt, flags=re.S).group(1)
+    m = re.search(r"class " + cls + r"\(ct\.Structure\):.*?\n\s+_fields_ = (.*?)\n", stub, flags=re.S)
+    assert m, cls
+    return re.findall(r'"([A-Za-z_]+)"', m.group(1))
+
+
+def test_integration_stub_matches_the_abi_layout(mtr):
+    # the documented binding must pass the same box as the library's own
+    # (VolumeSize field order, MrcFileIOBaseFunction.h:57-70)
+    for cls, struct, ours in (("VolumeSize", "mtr_volume_size", mtr.VolumeSize),
+                              ("ProjectionSize", "mtr_projection_size", mtr.ProjectionSize)):
+        stub = _stub_fields(cls)
+        assert stub == [f[0] for f in ours._fields_], cls
+        assert stub == _header_fields(struct), cls
+
+
 def test_divide_slice_matches_reference_rule(mtr):
     # first Z % N parts carry one extra slice (MultiAxis.cpp:13-31)
     assert [mtr.divide_slice(i, 4, 10, 5) for i in range(4)] == [(5, 8), (8, 11), (11, 13), (13, 15)]
