@@ -64,6 +64,28 @@ def test_continue_in_place_equals_one_solve(mtr, name, prec):
 
 
 @pytest.mark.parametrize("name,prec", PRECS, ids=[p[0] for p in PRECS])
+def test_consensus_difference_matches_an_independent_fp64_sum(mtr, name, prec):
+    # the early-stop quantity sum_a ||w_a - w-bar||^2 (the consensus of
+    # MultiAxis.cpp:61-66) against a sum formed in FP64 numpy from the
+    # stored state after every step, for every agent kind and storage type
+    pb = problem(mtr, seed=36)
+    say("consensus difference vs FP64 numpy", name)
+    with load(mtr, pb, prec) as c:
+        c.mace_init()
+        for step in range(4):
+            c.mace_continue(1, 2, 0.5, 0.5, out=False)
+            wbar = c.get_volume(mtr.VOL_WBAR).astype(np.float64)
+            ref = 0.0
+            for a in range(pb.K):
+                d = c.get_volume(mtr.VOL_W, a).astype(np.float64) - wbar
+                ref += float(np.dot(d.ravel(), d.ravel()))
+            got = c.consensus_diff2()
+            say(f"  step {step + 1}: device {got:.17e}, numpy {ref:.17e}")
+            assert ref > 0
+            assert abs(got - ref) <= 1e-12 * ref
+
+
+@pytest.mark.parametrize("name,prec", PRECS, ids=[p[0] for p in PRECS])
 def test_resume_from_checkpoint_in_a_new_context(mtr, name, prec):
     pb = problem(mtr, seed=32)
     say("resume from checkpoint", name)
