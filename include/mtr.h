@@ -232,6 +232,21 @@ int mtr_consensus_diff2(mtr_ctx *ctx, double *out);
  * gather. */
 int mtr_op_fp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
               const double *xc, const double *yc, const float *x, float *s, float *c);
+/* Allocation-free variant of mtr_op_fp, on device buffers: x_dev (vol box,
+ * [Z][Y][X]) in, s_dev and c_dev ([AngN][Y][X]) out, all float device
+ * memory of the context's GPU (MTR_ERR_ARG otherwise).  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc or hipFree: the view tables go to a
+ * fixed per-GPU scratch array of the library (61 KB, held until the process
+ * ends), so at most MTR_NOALLOC_MAX_VIEWS views per call (MTR_ERR_SHAPE
+ * beyond).  Calls that overlap on one GPU (from several host threads or
+ * contexts) share that array and race; calls in sequence do not.  Returns
+ * after the projection has finished.  Same kernel and results as
+ * mtr_op_fp. */
+#define MTR_NOALLOC_MAX_VIEWS 256
+int mtr_op_fp_noalloc(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol,
+                      const double *xc, const double *yc, const float *x_dev, float *s_dev, float *c_dev);
 /* r = p - (c != 0 ? s / c : s). */
 int mtr_op_residual(mtr_ctx *ctx, int64_t n, const float *p, const float *s, const float *c,
                     float *r);

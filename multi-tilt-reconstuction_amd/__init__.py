@@ -279,6 +279,13 @@ class Context:
         )
         return s, c
 
+    def op_fp_noalloc(self, prj: ProjectionSize, vol: VolumeSize, xc, yc, x_dev: int, s_dev: int, c_dev: int) -> None:
+        """Forward projection on device buffers (addresses as ints) without
+        any device allocation; see mtr_op_fp_noalloc in include/mtr.h."""
+        xc, yc = _f64(xc), _f64(yc)
+        _check(lib.mtr_op_fp_noalloc(self._h, _ct.byref(prj), _ct.byref(vol), _dp(xc), _dp(yc), x_dev, s_dev, c_dev),
+               "op_fp_noalloc")
+
     def op_residual(self, p, s, c):
         p, s, c = _f32(p), _f32(s), _f32(c)
         r = _np.empty_like(p)

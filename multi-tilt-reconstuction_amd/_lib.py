@@ -75,6 +75,7 @@ _PROTOS = {
     "mtr_residual_norm2": (_i, [_v, _i, _pd]),
     "mtr_consensus_diff2": (_i, [_v, _pd]),
     "mtr_op_fp": (_i, [_v, _pprj, _pvol, _pd, _pd, _pf, _pf, _pf]),
+    "mtr_op_fp_noalloc": (_i, [_v, _pprj, _pvol, _pd, _pd, _v, _v, _v]),
     "mtr_op_residual": (_i, [_v, _i64, _pf, _pf, _pf, _pf]),
     "mtr_op_bp": (_i, [_v, _pprj, _pvol, _pd, _pd, _pf, _pf, _f]),
     "mtr_op_bpt": (_i, [_v, _pprj, _pvol, _pd, _pd, _pf, _pf]),

@@ -1382,6 +1382,8 @@ int copy_volume_in(mtr_ctx *ctx, void *dev, const float *host)
 
 // Device tables of a context: the agent table in agent order and sorted
 // by kernel kind, and the consensus partial sums.
+int scratch_init(int device);  // the allocation-free entry point's per-device scratch
+
 int alloc_tables(mtr_ctx *c)
 {
     if (hipMalloc(&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
@@ -1416,6 +1418,10 @@ int mtr_ctx_create(int device, mtr_ctx **out)
         return fail(MTR_ERR_HIP, "stream creation failed");
     }
     if (const int rc = alloc_tables(ctx); rc != MTR_OK) {
+        mtr_ctx_destroy(ctx);
+        return rc;
+    }
+    if (const int rc = scratch_init(device); rc != MTR_OK) {
         mtr_ctx_destroy(ctx);
         return rc;
     }
@@ -2087,6 +2093,125 @@ int mtr_op_fp(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     HIP_TRY(hipMemcpyAsync(c, t.c.p, sizeof(float) * P, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return collect_timing(ctx);
+}
+
+// ---- allocation-free forward projection --------------------------------
+// mtr_op_fp keeps the agent record and the per-view tables (coefficients and
+// ray plans) in buffers it allocates per call; mtr_op_fp_noalloc stages them
+// in `scratch`, one fixed array per GPU held for the life of the process.
+// Host code never uses the array's name as an address: the address of each
+// device's copy comes from hipGetSymbolAddress, taken once per device with
+// that device current (at context creation, or at the entry point's first
+// call on the device) and kept in g_scratch_ptr.
+namespace {
+constexpr size_t round256(size_t n) { return (n + 255) / 256 * 256; }
+constexpr size_t kNaCoefOff = round256(sizeof(AgentDev));
+constexpr size_t kNaPlanOff = kNaCoefOff + round256(sizeof(ViewCoef) * MTR_NOALLOC_MAX_VIEWS);
+constexpr size_t kNaBytes = kNaPlanOff + round256(sizeof(FpPlan) * MTR_NOALLOC_MAX_VIEWS);
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kNaMaxDevices];
+
+// Address of this device's copy of `scratch`; `device` must be current.
+int scratch_init(int device)
+{
+    if (device < 0 || device >= kNaMaxDevices) return fail(MTR_ERR_RANGE, "device %d: no scratch slot", device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return MTR_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(MTR_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char *>(p);
+    return MTR_OK;
+}
+
+unsigned char *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kNaMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+
+// MTR_OK when q is device memory of `device`; clears the error state a
+// failed query leaves, so the next launch does not report it.
+int check_device_buffer(const void *q, int device, const char *what)
+{
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, q);
+    if (e != hipSuccess) (void)hipGetLastError();
+    if (e != hipSuccess || at.type != hipMemoryTypeDevice || at.device != device)
+        return fail(MTR_ERR_ARG, "%s is not device memory of device %d", what, device);
+    return MTR_OK;
+}
+}  // namespace
+
+int mtr_op_fp_noalloc(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_size *vol, const double *xc,
+                      const double *yc, const float *x_dev, float *s_dev, float *c_dev)
+{
+    if (!ctx || !prj || !vol || !xc || !yc || !x_dev || !s_dev || !c_dev) return fail(MTR_ERR_ARG, "null argument");
+    if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
+    if (!vol_valid(vol)) return fail(MTR_ERR_SHAPE, "invalid volume size");
+    const int A = prj->AngN;
+    if (A > MTR_NOALLOC_MAX_VIEWS)
+        return fail(MTR_ERR_SHAPE, "%d views: the allocation-free forward projection takes at most %d", A,
+                    MTR_NOALLOC_MAX_VIEWS);
+    MTR_TRY(set_device(ctx));
+    unsigned char *base = scratch_ptr(ctx->device);
+    if (!base) {
+        MTR_TRY(scratch_init(ctx->device));
+        base = scratch_ptr(ctx->device);
+    }
+    MTR_TRY(check_device_buffer(x_dev, ctx->device, "x"));
+    MTR_TRY(check_device_buffer(s_dev, ctx->device, "s"));
+    MTR_TRY(check_device_buffer(c_dev, ctx->device, "c"));
+    // host staging on the stack (no heap): 61 KB at the view maximum
+    ViewCoef coef[MTR_NOALLOC_MAX_VIEWS];
+    FpPlan plan[MTR_NOALLOC_MAX_VIEWS];
+    bool lanes_k = true;
+    for (int v = 0; v < A; v++) {
+        for (int m = 0; m < 4; m++) {
+            coef[v].c[m] = xc[4 * v + m];
+            coef[v].c[4 + m] = yc[4 * v + m];
+        }
+        const int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
+        if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
+        lanes_k = lanes_k && plan[v].kind == 1 && plan[v].W == 0;
+    }
+    AgentDev d{};
+    d.nu = prj->X;
+    d.nv = prj->Y;
+    d.A = A;
+    d.Xs = vol->Xstart;
+    d.Xe = vol->Xend;
+    d.Ys = vol->Ystart;
+    d.Ye = vol->Yend;
+    d.Zs = vol->Zstart;
+    d.Ze = vol->Zend;
+    d.X = vol->X;
+    d.Y = vol->Y;
+    d.Z = vol->Z;
+    d.coef = reinterpret_cast<const ViewCoef *>(base + kNaCoefOff);
+    d.plan = reinterpret_cast<const FpPlan *>(base + kNaPlanOff);
+    d.pk = prj->X;
+    d.pi = 1;
+    d.x = const_cast<float *>(x_dev);
+    d.sx = 1;
+    d.sy = vol->X;
+    d.sz = (int64_t)vol->X * vol->Y;
+    // every pixel of s and c is written by the launch (one thread per
+    // pixel), so nothing in the scratch or the outputs needs clearing
+    HIP_TRY(hipMemcpyAsync(base + kNaCoefOff, coef, sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(base + kNaPlanOff, plan, sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(base, &d, sizeof d, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_fp(MTR_PREC_FP32, reinterpret_cast<const AgentDev *>(base), 1, fp_blocks(d, lanes_k), 1, lanes_k,
+                      s_dev, c_dev, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return MTR_OK;
 }
 
 int mtr_op_residual(mtr_ctx *ctx, int64_t n, const float *p, const float *s, const float *c, float *r)
