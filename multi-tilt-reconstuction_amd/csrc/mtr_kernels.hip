@@ -1564,6 +1564,19 @@ constexpr int kColElems = 7 * 64 * kColBpWaves;  // staged pairs per view (rows 
 
 int col_elems() { return kColElems; }
 
+// Runs of the column back-projection start at absolute multiples of rw, so
+// a box [Zs, Ze) is covered by the runs from col_run_base(Zs, rw) on; a box
+// that straddles a multiple of rw needs one run more than ceil(Z / rw).
+// Shared by the kernel and the host's grid size.
+__host__ __device__ inline int col_run_base(int Zs, int rw)
+{
+    return Zs >= 0 ? (Zs / rw) * rw : -(((-Zs) + rw - 1) / rw) * rw;
+}
+__host__ __device__ inline int col_runs(int Zs, int Ze, int rw)
+{
+    return (Ze - col_run_base(Zs, rw) + rw - 1) / rw;
+}
+
 template <class T, int RW, int MODE>
 __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__restrict__ ags, float lambda)
 {
@@ -1578,8 +1591,8 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
     const int s1 = p1y ? ag.Ys : ag.Xs, s2 = p1y ? ag.Xs : ag.Ys;
     // runs start at absolute multiples of RW, so that a voxel's run (and
     // its arithmetic) does not depend on the box, e.g. on a Z-slab split
-    const int zal = ag.Zs >= 0 ? (ag.Zs / RW) * RW : -(((-ag.Zs) + RW - 1) / RW) * RW;
-    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kColBpWaves - 1) / kColBpWaves, ntz = (ag.Ze - zal + RW - 1) / RW;
+    const int zal = col_run_base(ag.Zs, RW);
+    const int nt1 = (n1 + 63) / 64, nt2 = (n2 + kColBpWaves - 1) / kColBpWaves, ntz = col_runs(ag.Zs, ag.Ze, RW);
     const int nb = nt1 * nt2 * ntz, per = (nb + 7) / 8;
     const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
@@ -2507,7 +2520,7 @@ hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int block
 int bp_col_blocks(const AgentDev &ag, int rw)
 {
     const int n1 = ag.col_p1y ? ag.Y : ag.X, n2 = ag.col_p1y ? ag.X : ag.Y;
-    const int nb = ((n1 + 63) / 64) * ((n2 + kColBpWaves - 1) / kColBpWaves) * ((ag.Z + rw - 1) / rw);
+    const int nb = ((n1 + 63) / 64) * ((n2 + kColBpWaves - 1) / kColBpWaves) * col_runs(ag.Zs, ag.Ze, rw);
     return ((nb + 7) / 8) * 8;
 }
 

@@ -143,6 +143,10 @@ AGENT_GEOMS = GEOMS + [
     ("col60_deep", dict(phis=[60.0], X=40, Y=36, Z=75)),           # two 64-plane chunks per column
     ("col135_small_det", dict(phis=[135.0], X=30, Y=30, Z=20, nu=17, nv=13)),  # footprint off the detector
     ("col45_origin", dict(phis=[45.0], origin=(3, -2, 5))),
+    # boxes whose Z range straddles a multiple of the 16-plane back-projection
+    # runs (one run more than ceil(Z / 16)), wide enough for 24+ tiles
+    ("col45_straddle", dict(phis=[45.0], X=128, Y=96, Z=8, origin=(0, 0, 12))),
+    ("col135_negz", dict(phis=[135.0], X=96, Y=128, Z=10, origin=(-4, 3, -5))),
     ("col75_wide", dict(phis=[75.0], X=150, Y=21, Z=9)),            # several back-projection tiles
     # deep columns, tilts in 5-degree steps: forward-projection chunks read
     # in groups of 4 (Z >= 256) and 2 (Z >= 128), lanes owning 4 planes
