@@ -274,7 +274,16 @@ enum {
     MTR_K_NORMALISER = 4,/* geometry-only FP normaliser                     */
     MTR_K_ELEMENTWISE = 5,/* standalone z / Mann / mean / residual          */
     MTR_K_SPLAT = 6,     /* plain form: atomic splat FP                     */
-    MTR_K_COUNT = 7
+    /* Per-kernel parts of the classes above, counted in addition to them
+     * (SIRT iterations only, not the initial BPT):                        */
+    MTR_K_FP_STREAM = 7,     /* row-separable streaming FP + residual      */
+    MTR_K_FP_COLUMN = 8,     /* column-plan FP into partial rows           */
+    MTR_K_FP_COLUMN_RES = 9, /* column-plan row sums + residual            */
+    MTR_K_FP_GENERAL = 10,   /* general-plan FP + residual                 */
+    MTR_K_BP_SEPARABLE = 11, /* runs along the plane axis                  */
+    MTR_K_BP_COLUMN = 12,    /* runs along Z                               */
+    MTR_K_BP_GENERAL = 13,   /* general plan                               */
+    MTR_K_COUNT = 14
 };
 /* Enable (1) or disable (0) per-launch event timing. */
 int mtr_timing_enable(mtr_ctx *ctx, int on);

@@ -438,9 +438,9 @@ struct Timed {
     mtr_ctx *ctx;
     int kernel;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mtr_ctx *c, int k) : ctx(c), kernel(k)
+    Timed(mtr_ctx *c, int k, bool on = true) : ctx(c), kernel(k)
     {
-        if (ctx->timing) {
+        if (ctx->timing && on) {
             a = take_event(ctx);
             b = take_event(ctx);
             if (a && b) (void)hipEventRecord(a, ctx->stream);
@@ -627,11 +627,17 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
         }
         Timed t(eng, MTR_K_FP);
         if (a0.col.on) {
-            HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
+            {
+                Timed tk(eng, MTR_K_FP_COLUMN);
+                HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
+            }
+            Timed tk(eng, MTR_K_FP_COLUMN_RES);
             HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
         } else if (a0.fp_nc) {
+            Timed tk(eng, MTR_K_FP_STREAM);
             HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, eng->stream));
         } else {
+            Timed tk(eng, MTR_K_FP_GENERAL);
             HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, eng->stream));
         }
         s = e;
@@ -657,6 +663,7 @@ int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float
                                     : (m.sep ? bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw) : bp_blocks(d, eng->bp_tz)));
         }
         Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP);
+        Timed tk(eng, cm.on ? MTR_K_BP_COLUMN : (m.sep ? MTR_K_BP_SEPARABLE : MTR_K_BP_GENERAL), !bpt);
         if (cm.on) {
             HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, eng->stream));
         } else if (m.sep) {
