@@ -1,16 +1,35 @@
 #!/usr/bin/env python3
 """Benchmark: MACE reconstructions per second on one MI355X.
 
-A step is one whole reconstruction of the dual-axis MACE workload
-(512x512x128 volume, two tilt series at in-plane axes 0 and 90 degrees,
-61 views each over +-60 degrees on a 512x512 detector, 1% noise, 10 MACE
-steps of 5 SIRT iterations, lambda 0.5, rho 0.5): BPT of both series, the
-summed initial model, then 10 x (2 x 5 SIRT iterations + fused consensus).
-Inputs are synthetic (seeded ellipsoid phantom, analytic ray sums) and are
-resident in HBM before the timed region starts.
+Default workload (--config c3), the largest that fits one GPU: four-axis
+MACE on a 1024x1024x256 volume, tilt series at in-plane axes 0/45/90/135
+deg, 61 views each over +-60 deg on a 1024x1024 detector, 1% noise, 20 MACE
+steps of 10 SIRT iterations, lambda 0.5, rho 0.5.  Inputs are synthetic
+(seeded ellipsoid phantom, analytic ray sums) and resident in HBM before
+anything is timed.
 
-Prints one JSON line.  Per-kernel times come from HIP events recorded on
-the library's own stream around every launch inside the timed region.
+A timed step is one MACE step: one replay of the captured HIP graph that
+holds 10 SIRT iterations of all four agents (each kernel kind launched once
+per iteration over its agents) and the fused consensus (z, Mann, mean, next
+z, FP64 consensus difference).  The initial model (BPT of every agent and
+their sum) is timed on its own, after a first initialisation that tunes the
+tiles, so
+
+    value = reconstructions/s = 1 / (t_init + M * t_step).
+
+Per-kernel times come from HIP events recorded on the library's own stream
+around every launch of one further, host-issued MACE step.  The roofline is
+taken for the kernel with the largest share of that step: FP32 FLOP (8 per
+voxel-view update) against the FP32 vector peak, with its HBM fraction on
+algorithmic bytes and the rocprof traffic per launch beside it.
+
+Comparison points, each labelled in the output: the same gather kernels
+driven one agent at a time with a host sync after every SIRT iteration and
+no graph (per-agent loop), the reference's structure with float-atomic
+splats (plain loop), and the CPU oracle on the host cores (timed on Z slabs
+and extrapolated linearly in depth).
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -18,6 +37,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -26,58 +46,193 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
-# FP32 vector lanes: 256 CU x 64 lanes x 2.4 GHz (non-packed issue).
-LANE_OPS_PEAK = 256 * 64 * 2.4e9
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, packed FMA
+FP32_LANE_OPS = 256 * 64 * 2.4e9  # non-packed issue: 39.3 T lane-ops/s (78.6 TFLOP/s of FMA)
+FLOP_PER_VVU = 8                  # 4 weight x value multiply-adds per bilinear 4-tap evaluation
+HIP_KERNEL = {"fp_stream": "k_fp_stream", "fp_column": "k_fp_col", "fp_column_res": "k_fp_col_res",
+              "fp_general": "k_fp", "bp_separable": "k_bp_sep", "bp_column": "k_bp_col", "bp_general": "k_bp"}
+AXES = {1: "single-axis", 2: "dual-axis", 3: "three-axis", 4: "four-axis"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pb, threads: int):
-    """Times the CPU oracle on a bounded sample and scales to recon/s."""
+def cpu_model() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def slab(mtr, pb, nz):
+    """The central nz planes of the workload's box, same geometry."""
+    z0 = pb.vol.Zstart + (pb.vol.Z - nz) // 2
+    return mtr.volume_size(pb.vol.X, pb.vol.Y, nz, pb.vol.Xstart, pb.vol.Ystart, z0)
+
+
+def oracle_slab_times(mtr, orc, pb, agents, depths, keep=None):
+    """Oracle BPT and one SIRT iteration of each agent on central Z slabs of
+    each depth; returns {agent: {depth: (t_bpt, t_sirt)}} and the outputs of
+    the `keep` depth (BPT, SIRT1) per agent."""
+    times, outs = {}, {}
+    for a in agents:
+        times[a] = {}
+        for nz in depths:
+            vol = slab(mtr, pb, nz)
+            t0 = time.perf_counter()
+            x = orc.bpt(pb.prjs[a], vol, pb.xcs[a], pb.ycs[a], pb.ps[a])
+            t1 = time.perf_counter()
+            xs = orc.sirt(pb.prjs[a], vol, pb.xcs[a], pb.ycs[a], pb.ps[a], x, 1, pb.lam)
+            t2 = time.perf_counter()
+            times[a][nz] = (t1 - t0, t2 - t1)
+            if nz == keep:
+                outs[a] = (x, xs)
+    return times, outs
+
+
+def extrapolate(times, depths, Z):
+    """t(nz) = alpha + beta * nz through the two depths, at the full depth Z
+    (the per-pixel work of a SIRT iteration does not grow with depth)."""
+    d0, d1 = depths
+    full = {}
+    for a, td in times.items():
+        f = []
+        for q in range(2):
+            beta = (td[d1][q] - td[d0][q]) / (d1 - d0)
+            alpha = td[d0][q] - beta * d0
+            f.append(max(alpha + beta * Z, td[d1][q]))
+        full[a] = tuple(f)
+    return full
+
+
+def consensus_time(orc, pb, nzc):
+    """One consensus update (z and Mann of every agent, then the mean) on an
+    nzc-plane slab, scaled by volume (element-wise work)."""
+    shape = (nzc, pb.vol.Y, pb.vol.X)
+    rng = np.random.default_rng(0)
+    ws = [rng.standard_normal(shape, dtype=np.float32) for _ in range(pb.K)]
+    wb = rng.standard_normal(shape, dtype=np.float32)
+    v = rng.standard_normal(shape, dtype=np.float32)
+    t0 = time.perf_counter()
+    for a in range(pb.K):
+        z = orc.form_z(ws[a], wb)
+        ws[a] = orc.mann(v, ws[a], z, pb.rho)
+    orc.mean(ws)
+    return (time.perf_counter() - t0) * pb.vol.Z / nzc
+
+
+def cpu_baseline(mtr, pb, threads: int):
+    """The CPU oracle (the reference's arithmetic, C + OpenMP) timed on the
+    host cores on a bounded sample of the workload: BPT and one SIRT
+    iteration of every agent on central Z slabs of two depths, extrapolated
+    linearly to the full depth, plus one consensus update on a slab scaled
+    by volume; and the same per-agent sample with one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the CPU restatement (timed baseline only)
 
-    used = orc.set_threads(threads) if threads > 0 else orc.set_threads(0)
-    # BPT + 1 SIRT iteration of agents 0, 1, ... until about 10 s of work
-    # (every agent of the two-agent workload), then one consensus update
-    t_bpt = t_sirt = 0.0
-    n = 0
-    for a in range(pb.K):
-        prj, vol, xc, yc, p = pb.prjs[a], pb.vol, pb.xcs[a], pb.ycs[a], pb.ps[a]
-        t0 = time.perf_counter()
-        xa = orc.bpt(prj, vol, xc, yc, p)
-        t_bpt += time.perf_counter() - t0
-        t0 = time.perf_counter()
-        xsa = orc.sirt(prj, vol, xc, yc, p, xa, 1, pb.lam)
-        t_sirt += time.perf_counter() - t0
-        if a == 0:
-            x, xs = xa, xsa
-        n += 1
-        if t_bpt + t_sirt >= 10.0:
-            break
-    w = x.copy()
+    K, M, S, Z = pb.K, pb.M, pb.S, pb.vol.Z
+    big = Z >= 128
+    depths = (4, 8) if big else (max(1, Z // 8), max(2, Z // 4))
+    used = orc.set_threads(threads)
     t0 = time.perf_counter()
-    z = orc.form_z(w, x)
-    w = orc.mann(xs, w, z, pb.rho)
-    wb = orc.mean([w, x])
-    t_cons = time.perf_counter() - t0
-    K, M, S = pb.K, pb.M, pb.S
-    # per-agent averages over the sampled agents; consensus per step:
-    # K x (z + Mann) + mean, the sample above did one of each
-    t_recon = K * t_bpt / n + M * K * S * t_sirt / n + M * K * t_cons
-    del wb
+    times, outs = oracle_slab_times(mtr, orc, pb, range(K), depths, keep=depths[1])
+    full = extrapolate(times, depths, Z)
+    nzc = min(Z, 16)
+    t_cons = consensus_time(orc, pb, nzc)
+    t_all = time.perf_counter() - t0
+    t_recon = sum(f[0] for f in full.values()) + M * S * sum(f[1] for f in full.values()) + M * t_cons
+    # one thread: the oblique agent (or the only one) on thinner slabs
+    a1 = 1 if K > 1 else 0
+    d1 = (1, 2) if big else depths
+    orc.set_threads(1)
+    t0 = time.perf_counter()
+    times1, _ = oracle_slab_times(mtr, orc, pb, [a1], d1)
+    full1 = extrapolate(times1, d1, Z)[a1]
+    t_cons1 = consensus_time(orc, pb, nzc)
+    t_one = time.perf_counter() - t0
+    orc.set_threads(threads)
+    t_recon1 = K * full1[0] + M * S * K * full1[1] + M * t_cons1
+    per_agent = {str(a): {"bpt_s": f[0], "sirt_iter_s": f[1]} for a, f in full.items()}
     return {
         "value": 1.0 / t_recon,
         "unit": "reconstructions/s",
         "cores": used,
         "kind": "port",
-        "sample": (f"oracle BPT + 1 SIRT iteration of {n} of {K} agents and 1 consensus update at full size "
-                   f"({t_bpt:.2f}+{t_sirt:.2f}+{t_cons:.2f} s), scaled per agent to {K} BPT + {M*K*S} SIRT "
-                   f"iterations + {M} consensus steps"),
-        "seconds_sample": t_bpt + t_sirt + t_cons,
-    }, (x, xs)
+        "sample": (f"oracle BPT + 1 SIRT iteration of each of the {K} agents on central Z slabs of depth "
+                   f"{depths[0]} and {depths[1]} of the full {pb.vol.X}x{pb.vol.Y} box and detector, extrapolated "
+                   f"linearly to depth {Z}; one consensus update on a {nzc}-plane slab scaled by volume; "
+                   f"reconstruction = {K} BPT + {M * K * S} SIRT iterations + {M} consensus steps "
+                   f"({t_all:.1f} s of CPU work with {used} threads)"),
+        "seconds_per_reconstruction": t_recon,
+        "per_agent_full_size": per_agent,
+        "consensus_step_s": t_cons,
+        "threads": used,
+        "nproc": os.cpu_count(),
+        "affinity_cores": len(os.sched_getaffinity(0)),
+        "cpu_model": cpu_model(),
+        "one_core": {
+            "value": 1.0 / t_recon1,
+            "unit": "reconstructions/s",
+            "seconds_per_reconstruction": t_recon1,
+            "sample": (f"1 thread: oracle BPT + 1 SIRT iteration of agent {a1} on slabs of depth {d1[0]} and "
+                       f"{d1[1]}, extrapolated to depth {Z}, taken for all {K} agents; consensus on the "
+                       f"{nzc}-plane slab scaled by volume ({t_one:.1f} s of CPU work)"),
+        },
+        "seconds_sample": t_all + t_one,
+    }, (depths[1], outs)
+
+
+def slab_parity(mtr, pb, nz, outs):
+    """GPU BPT + 1 SIRT iteration of every agent on the same central slab the
+    CPU sample ran, against the oracle's outputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the CPU checker
+
+    vol = slab(mtr, pb, nz)
+    res = {}
+    with mtr.Context(0) as c:
+        c.set_volume(vol)
+        for a in range(pb.K):
+            c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+        for a in range(pb.K):
+            g_bpt = c.bpt(a)
+            g_s1 = c.sirt(a, 1, pb.lam)
+            x, xs = outs[a]
+            res[str(a)] = {"bpt_rel_l2": orc.rel_l2(g_bpt, x), "sirt1_rel_l2": orc.rel_l2(g_s1, xs)}
+    return {"slab": f"{vol.X}x{vol.Y}x{vol.Z} (central planes), full detector", "agents": res}
+
+
+def fixture_parity(mtr, ctx, pb, config):
+    """w-bar of a one-step solve (M=1, S=1) of the full workload against the
+    oracle fixture's sampled voxels, plane sums and norm."""
+    path = os.path.join(ROOT, "tests", "golden", f"large_{config}_m1s1.npz")
+    if not os.path.exists(path):
+        return None
+    g = np.load(path)
+    out = ctx.mace_solve(1, 1, pb.lam, pb.rho)
+    flat = out.reshape(-1)
+    idx = g["idx"]
+    planes = out.reshape(pb.vol.Z, -1).sum(axis=1, dtype=np.float64)
+    res = {"case": f"{config} full size, M=1 S=1", "sample_voxels": int(idx.size)}
+    for mode in ("ref", "f64"):
+        ref = g[mode].astype(np.float64)
+        d = flat[idx].astype(np.float64) - ref
+        res[f"sampled_rel_l2_vs_{mode}"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+        gp = g[mode + "_planes"]
+        res[f"plane_sums_rel_vs_{mode}"] = float(np.abs(planes - gp).max() / np.abs(gp).max())
+    return res
 
 
 def serving(mtr, args):
@@ -171,15 +326,107 @@ def load_traffic():
         return json.load(f)
 
 
+def kernel_table(mtr, ctx, pb, es, iters):
+    """Per-kernel launches, times, work and rates of a pass of `iters` SIRT
+    iterations of every agent: each kernel's work over the pass (its agents'
+    share) divided by its launch count gives the work per launch, whether a
+    launch covered one agent or several."""
+    N, A = pb.N, pb.prjs[0].AngN
+    kinds = [ctx.agent_kernels(a) for a in range(pb.K)]
+    n_fp = {"fp_stream": mtr.FPK_STREAM, "fp_column": mtr.FPK_COLUMN, "fp_column_res": mtr.FPK_COLUMN,
+            "fp_general": mtr.FPK_GENERAL}
+    n_bp = {"bp_separable": mtr.BPK_SEPARABLE, "bp_column": mtr.BPK_COLUMN, "bp_general": mtr.BPK_GENERAL}
+    table = {}
+    for k, name in enumerate(mtr.KERNEL_NAMES):
+        n, ms = ctx.timing_get(k)
+        if n == 0 or ms <= 0:
+            continue
+        e = {"launches": n, "avg_ms": ms / n, "total_ms": ms}
+        agents = None
+        if name in n_fp:
+            agents = [a for a in range(pb.K) if kinds[a][0] == n_fp[name]]
+        elif name in n_bp:
+            agents = [a for a in range(pb.K) if kinds[a][1] == n_bp[name]]
+        if agents:
+            Ps = [pb.P[a] for a in agents]
+            # algorithmic HBM bytes of one iteration (storage size es per element)
+            if name == "fp_column":
+                nbytes = es * N * len(agents)                   # read x (the partial rows are intermediates)
+            elif name == "fp_column_res":
+                nbytes = es * 2 * sum(Ps)                       # read p, write r
+            elif name.startswith("fp"):
+                nbytes = es * sum(N + 2 * P for P in Ps)        # read x, read p, write r
+            else:
+                nbytes = es * sum(2 * N + P for P in Ps)        # read r, read + write x
+            vvu = 0 if name == "fp_column_res" else N * A * len(agents)
+            # per launch: the pass's total over its launches
+            nbytes = nbytes * iters // n
+            vvu = vvu * iters // n
+            e.update({"hip_kernel": HIP_KERNEL[name], "agents_per_launch": len(agents) * iters / n,
+                      "bytes_per_launch": nbytes, "GBps": nbytes / (e["avg_ms"] * 1e-3) / 1e9,
+                      "hbm_frac": nbytes / (e["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS})
+            if vvu:
+                vps = vvu / (e["avg_ms"] * 1e-3)
+                e.update({"vvu_per_launch": vvu, "vvu_per_s": vps, "TFLOPs": vps * FLOP_PER_VVU / 1e12,
+                          "valu_frac": vps * FLOP_PER_VVU / 1e12 / FP32_PEAK_TFLOPS})
+        table[name] = e
+    if "consensus" in table:
+        c = table["consensus"]
+        nbytes = es * (4 * pb.K + 2) * N   # v, w, wbar in; w, next z, wbar out
+        c.update({"bytes_per_launch": nbytes, "GBps": nbytes / (c["avg_ms"] * 1e-3) / 1e9,
+                  "hbm_frac": nbytes / (c["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS})
+    return table
+
+
+def per_agent_step(mtr, ctx, pb):
+    """One MACE step with the same gather kernels driven one agent at a time:
+    z, S SIRT iterations with a host sync after each, Mann, then the mean;
+    no graph, no batching over agents."""
+    ctx.sync()
+    t0 = time.perf_counter()
+    for a in range(pb.K):
+        ctx.form_z(a)
+        for _ in range(pb.S):
+            ctx.sirt(a, 1, pb.lam, out=False)
+            ctx.sync()
+        ctx.mann(a, pb.rho)
+    ctx.mean()
+    ctx.sync()
+    return time.perf_counter() - t0
+
+
+def plain_recon(mtr, ctx, pb):
+    """The plain form (float-atomic splats, one agent at a time, the
+    normaliser recomputed every iteration, host syncs) timed on two short
+    solves, M=1 with S=1 and S=2; their difference is one SIRT iteration of
+    every agent, the rest the initial model and one consensus step."""
+    ts = []
+    for S in (1, 2):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.mace_solve(1, S, pb.lam, pb.rho, form=mtr.FORM_PLAIN, out=False)
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    it = ts[1] - ts[0]
+    init_and_step = ts[0] - it
+    t = init_and_step + pb.M * pb.S * it
+    return {"recon_per_s": 1.0 / t, "seconds_per_reconstruction": t, "sirt_iteration_all_agents_s": it,
+            "init_plus_one_consensus_s": init_and_step,
+            "sample": f"plain solves M=1 S=1 ({ts[0]:.2f} s) and M=1 S=2 ({ts[1]:.2f} s), scaled to "
+                      f"M={pb.M} S={pb.S} (consensus counted once)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
-    ap.add_argument("--plain-steps", type=int, default=1, help="plain per-agent loop reconstructions to time")
+    ap.add_argument("--steps", type=int, default=10, help="timed MACE steps")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed MACE steps before the timed ones")
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-loop comparison")
+    ap.add_argument("--no-per-agent", action="store_true", help="skip the per-agent-loop comparison")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="oracle threads (0: this process's cores, at most OMP_NUM_THREADS)")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0 = skip serving)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
@@ -197,15 +444,20 @@ def main() -> int:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU agent sharding is a "
               "design, not a build, here.", file=sys.stderr)
         return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
 
     mtr = importlib.import_module("multi-tilt-reconstuction_amd")
     t0 = time.perf_counter()
     pb = mtr.synth.config(args.config, threads=16)
     log(f"generated {args.config}: {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, K={pb.K} in {time.perf_counter()-t0:.1f}s")
     prec = {"fp32": mtr.PREC_FP32, "bf16": mtr.PREC_BF16, "fp16": mtr.PREC_FP16}[args.precision]
-    N = pb.N
-    Ps = pb.P
+    es = 4 if prec == mtr.PREC_FP32 else 2
     A = pb.prjs[0].AngN
+    K, M, S = pb.K, pb.M, pb.S
+    axes = AXES.get(K, f"{K}-axis")
+    phis = "/".join(f"{float(p):g}" for p in mtr.synth.CONFIGS[args.config]["phis"])
 
     ctx = mtr.Context(0, prec)
     copy_cal = None
@@ -214,111 +466,120 @@ def main() -> int:
         ms = ctx.calibrate_copy(nbytes)
         copy_cal = {"bytes": nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
     ctx.set_volume(pb.vol)
-    for a in range(pb.K):
+    for a in range(K):
         ctx.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
     ctx.sync()
 
-    for i in range(args.warmup):
-        ctx.mace_solve(pb.M, pb.S, pb.lam, pb.rho, out=False)
-        ctx.sync()
-        log(f"warmup {i} done")
+    # initial model: the first one tunes the tiles, the second is the cost
+    # every further reconstruction of this shape pays
+    t0 = time.perf_counter()
+    ctx.mace_init()
+    ctx.sync()
+    t_init_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ctx.mace_init()
+    ctx.sync()
+    t_init = time.perf_counter() - t0
+    log(f"initial model: {t_init_first*1e3:.1f} ms with tuning, {t_init*1e3:.1f} ms tuned")
 
-    # timed region: each MACE step replays one captured HIP graph
+    if args.warmup > 0:
+        ctx.mace_continue(args.warmup, S, pb.lam, pb.rho, out=False)
+        ctx.sync()
+        log(f"warmup: {args.warmup} MACE steps")
+
+    # timed region: `steps` MACE steps, each one replay of the captured graph
     ctx.sync()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        ctx.mace_solve(pb.M, pb.S, pb.lam, pb.rho, out=False)
+    _, ran = ctx.mace_continue(args.steps, S, pb.lam, pb.rho, out=False)
     ctx.sync()
     wall = time.perf_counter() - t0
-    log(f"timed {args.steps} reconstructions in {wall:.3f}s")
+    assert ran == args.steps
+    t_step = wall / args.steps
+    value = 1.0 / (t_init + M * t_step)
+    log(f"timed {args.steps} MACE steps in {wall:.3f}s ({t_step*1e3:.1f} ms/step): {value:.4f} reconstructions/s")
 
-    # per-kernel pass: HIP events around every launch (host-issued launches)
-    kt_steps = max(1, min(args.steps, 2))
+    # per-kernel pass: HIP events around every launch of one host-issued step
     ctx.timing_reset()
     ctx.timing(True)
     ctx.sync()
     t0 = time.perf_counter()
-    for i in range(kt_steps):
-        ctx.mace_solve(pb.M, pb.S, pb.lam, pb.rho, out=False)
+    ctx.mace_continue(1, S, pb.lam, pb.rho, out=False)
     ctx.sync()
-    wall_eager = time.perf_counter() - t0
+    t_step_eager = time.perf_counter() - t0
     ctx.timing(False)
-    log(f"per-kernel pass: {kt_steps} reconstructions in {wall_eager:.3f}s (host-issued launches)")
-
-    ktimes = {name: ctx.timing_get(k) for k, name in enumerate(mtr.KERNEL_NAMES)}
-    K, M, S = pb.K, pb.M, pb.S
-    steps = args.steps
-    ksteps = kt_steps
-    # Algorithmic HBM bytes per reconstruction (FP32 storage; 2-byte storage halves it).
-    es = 4 if prec == mtr.PREC_FP32 else 2
-    fp_bytes = sum(M * S * es * (N + 2 * P) for P in Ps)          # read x, read p, write r
-    bp_bytes = sum(M * S * es * (2 * N + P) for P in Ps)          # read r, read+write x
-    bpt_bytes = sum(es * (N + P) for P in Ps)
-    cons_bytes = M * es * (4 * K + 2) * N                         # v, w, wbar in; w, z, wbar out
-    vvu = {"fp": M * S * K * N * A, "bp": M * S * K * N * A, "bpt": K * N * A}
-    per = {}
-    for name, nbytes in (("fp", fp_bytes), ("bp", bp_bytes), ("bpt", bpt_bytes), ("consensus", cons_bytes)):
-        n, ms = ktimes[name]
-        if n == 0 or ms <= 0:
-            continue
-        per[name] = {
-            "launches": n,
-            "avg_ms": ms / n,
-            "total_ms": ms,
-            "bytes_per_launch": nbytes * ksteps / n,
-            "GBps": nbytes * ksteps / (ms * 1e-3) / 1e9,
-        }
-        if name in vvu:
-            per[name]["vvu_per_s"] = vvu[name] * ksteps / (ms * 1e-3)
-    dom = max(("fp", "bp"), key=lambda q: per.get(q, {}).get("total_ms", 0.0))
-    d = per[dom]
+    kernels = kernel_table(mtr, ctx, pb, es, S)
+    subs = [n for n in HIP_KERNEL if n in kernels]
+    dom = max(subs, key=lambda n: kernels[n]["total_ms"])
+    d = kernels[dom]
     traffic = load_traffic()
     tr = None
     if traffic and args.config in traffic and dom in traffic[args.config]:
         tr = traffic[args.config][dom]
+    step_kernel_ms = sum(kernels[n]["total_ms"] for n in subs) + kernels.get("consensus", {}).get("total_ms", 0.0)
     roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": d["GBps"],
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": d["GBps"] / HBM_PEAK_GBS,
+        "bound": "valu",
+        "kernel": d["hip_kernel"],
+        "achieved": d["TFLOPs"],
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": d["valu_frac"],
         "traffic": tr,
-        "bytes_per_launch": d["bytes_per_launch"],
+        "share_of_step": d["total_ms"] / step_kernel_ms,
+        "agents_per_launch": d["agents_per_launch"],
+        "vvu_per_launch": d["vvu_per_launch"],
+        "flop_per_launch": d["vvu_per_launch"] * FLOP_PER_VVU,
         "avg_launch_ms": d["avg_ms"],
-        "vvu_per_s": d.get("vvu_per_s"),
-        "note": ("at 61 views the projectors are bound on chip, not by HBM (counters: the streaming FP on the "
-                 "vector-memory data path, TA/TD about 90%; the back-projections on VALU, LDS and barriers); "
-                 "the HBM fraction is on algorithmic bytes"),
+        "vvu_per_s": d["vvu_per_s"],
+        "note": (f"FP32 FLOP at {FLOP_PER_VVU} per voxel-view update (2*N*A updates per SIRT iteration per agent) "
+                 f"against the packed-FMA FP32 peak; packed FP32 is disabled in this build, so the issue limit is "
+                 f"{FP32_LANE_OPS/1e12:.1f} T lane-ops/s ({2*FP32_LANE_OPS/1e12:.1f} TFLOP/s of FMA). At 61 "
+                 f"views the projectors are bound on chip (VALU, LDS, vector-memory data path), not by HBM"),
+        "hbm": {"achieved": d["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
+                "bytes_per_launch": d["bytes_per_launch"], "basis": "algorithmic bytes of the kernel"},
     }
-    if "consensus" in per:
-        roofline["consensus_GBps"] = per["consensus"]["GBps"]
-        roofline["consensus_frac"] = per["consensus"]["GBps"] / HBM_PEAK_GBS
+    if "consensus" in kernels:
+        roofline["consensus_hbm"] = {"kernel": "k_consensus", "achieved": kernels["consensus"]["GBps"],
+                                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kernels["consensus"]["hbm_frac"],
+                                     "bytes_per_launch": kernels["consensus"]["bytes_per_launch"]}
+    log(f"dominant kernel {d['hip_kernel']}: {d['avg_ms']:.2f} ms/launch, {d['TFLOPs']:.2f} TFLOP/s "
+        f"({100*d['valu_frac']:.1f}% of FP32 peak)")
+
+    per_agent = None
+    if not args.no_per_agent:
+        tpa = per_agent_step(mtr, ctx, pb)
+        ra = 1.0 / (t_init + M * tpa)
+        per_agent = {"ms_per_mace_step": tpa * 1e3, "recon_per_s": ra,
+                     "form": "same gather kernels and tiles, one agent at a time, host sync after every SIRT "
+                             "iteration, no graph; reconstructions/s = 1 / (t_init + M * step)"}
+        log(f"per-agent loop: {tpa*1e3:.1f} ms per MACE step")
 
     plain = None
-    if args.plain_steps > 0 and prec == mtr.PREC_FP32:
-        ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(args.plain_steps):
-            ctx.mace_solve(M, S, pb.lam, pb.rho, form=mtr.FORM_PLAIN, out=False)
-        ctx.sync()
-        tp = (time.perf_counter() - t0) / args.plain_steps
-        plain = {"recon_per_s": 1.0 / tp, "ms_per_recon": tp * 1e3}
-        log(f"plain loop: {tp*1e3:.1f} ms per reconstruction")
+    if not args.no_plain and prec == mtr.PREC_FP32:
+        plain = plain_recon(mtr, ctx, pb)
+        log(f"plain loop: {plain['seconds_per_reconstruction']:.1f} s per reconstruction ({plain['sample']})")
+
+    parity = {}
+    if prec == mtr.PREC_FP32:
+        fx = fixture_parity(mtr, ctx, pb, args.config)
+        if fx:
+            parity["full_size"] = fx
+            log(f"full-size parity: {fx}")
+    ctx.close()
 
     cpu = None
-    parity = None
     if not args.no_cpu:
-        log("timing the CPU oracle sample ...")
-        cpu, (x_bpt, x_s1) = cpu_baseline(pb, args.cpu_threads)
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as orc
-
-        g_bpt = ctx.bpt(0)
-        g_s1 = ctx.sirt(0, 1, pb.lam)
-        parity = {"bpt_rel_l2": orc.rel_l2(g_bpt, x_bpt), "sirt1_rel_l2": orc.rel_l2(g_s1, x_s1)}
-        log(f"parity sample: {parity}")
-    ctx.close()
+        threads = args.cpu_threads
+        if threads <= 0:
+            threads = len(os.sched_getaffinity(0))
+            if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+                threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
+        log(f"timing the CPU oracle sample with {threads} threads ...")
+        cpu, (nz, outs) = cpu_baseline(mtr, pb, threads)
+        log(f"cpu baseline: {cpu['seconds_per_reconstruction']:.0f} s per reconstruction with {cpu['threads']} "
+            f"threads, {cpu['one_core']['seconds_per_reconstruction']:.0f} s with one")
+        if prec == mtr.PREC_FP32:
+            parity["slab_sample"] = slab_parity(mtr, pb, nz, outs)
+            log(f"slab parity: {parity['slab_sample']}")
 
     serve = None
     if args.serve_requests > 0 and prec == mtr.PREC_FP32:
@@ -326,34 +587,38 @@ def main() -> int:
         log(f"serving: p50 {serve['p50_latency_s']:.2f}s p99 {serve['p99_latency_s']:.2f}s, "
             f"{serve['throughput_req_per_s']:.2f} req/s")
 
-    value = steps / wall
-    axes = {1: "single-axis", 2: "dual-axis", 3: "three-axis", 4: "four-axis"}.get(pb.K, f"{pb.K}-axis")
     out = {
-        "metric": f"MACE reconstructions/sec ({axes} {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, {pb.K}x{A} views, "
-                  f"{pb.M} MACE x {pb.S} SIRT)",
+        "metric": f"MACE reconstructions/sec ({axes} {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, {K}x{A} views, "
+                  f"{M} MACE x {S} SIRT; a step is one MACE step, value = 1/(t_init + {M} x step))",
         "value": value,
         "unit": "reconstructions/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / steps * 1e3,
-        "ms_per_step_host_issued": wall_eager / kt_steps * 1e3,
+        "ms_per_step": t_step * 1e3,
+        "ms_per_step_host_issued": t_step_eager * 1e3,
+        "init_ms": t_init * 1e3,
+        "init_ms_with_tuning": t_init_first * 1e3,
+        "seconds_per_reconstruction": t_init + M * t_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == mtr.PREC_FP32 else ("bf16" if prec == mtr.PREC_BF16 else "f16"),
         "data": "synthetic (seeded ellipsoid phantom, analytic ray sums, 1% noise)",
         "config": {
-            "workload": f"{args.config}: dual-axis MACE {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, K={K}, {A} views/axis, "
-                        f"{pb.prjs[0].X}x{pb.prjs[0].Y} detector, M={M}, S={S}, lambda={pb.lam}, rho={pb.rho}",
+            "workload": f"{args.config}: {axes} MACE {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, K={K} (in-plane axes "
+                        f"{phis} deg), {A} views/axis over +-60 deg, {pb.prjs[0].X}x{pb.prjs[0].Y} detector, "
+                        f"M={M}, S={S}, lambda={pb.lam}, rho={pb.rho}",
             "parallelism": "single GPU, all agents resident",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": per,
+        "kernels": kernels,
+        "per_agent_loop": per_agent,
         "plain_loop": plain,
+        "speedup_vs_per_agent": (value / per_agent["recon_per_s"]) if per_agent else None,
         "speedup_vs_plain": (value / plain["recon_per_s"]) if plain else None,
-        "parity_sample": parity,
+        "parity": parity or None,
         "serving": serve,
         "copy_calibration": copy_cal,
     }

@@ -1,0 +1,95 @@
+#!/usr/bin/env python3
+"""Oracle fixtures for solves too large to run the CPU oracle inside a GPU test.
+
+For each case the seeded synthetic problem is generated with the package's
+generator (the same code the GPU test calls), the CPU oracle runs the
+reference's BPT-sum initialisation and MACE loop (MultiAxis.cpp:40-71,
+163-200 around FunctionOnGPU.cu:212-244) in both of its modes (`ref`, the
+reference's mixed FP64/FP32 precision, and `f64`), and a seeded sample of
+output voxels is written with per-Z-plane sums and the squared norm:
+
+  tests/golden/large_<case>.npz
+    idx        int64 [n]   sampled flat voxel indices ([Z][Y][X] order)
+    ref, f64   values of w-bar at idx (float32 / float64)
+    ref_planes, f64_planes   FP64 sum of each Z plane of w-bar
+    ref_norm2, f64_norm2     FP64 sum of squares of w-bar
+    p_sha256   hash of every agent's projections (the test checks that it
+               regenerated the same inputs before comparing)
+
+Run from the repository root (about 50 minutes on 8 cores, 20 GB of memory):
+    python tests/golden/make_large_fixtures.py [case ...]
+"""
+from __future__ import annotations
+
+import hashlib
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# name -> (config, overrides of its sizes, M, S); lambda and rho are the config's
+CASES = {
+    # the four-axis workload (axes 0/45/90/135, 1024x1024x256, 61 views) at one MACE step of one SIRT iteration
+    "c3_m1s1": ("c3", {}, 1, 1),
+    # the dual-axis workload's full schedule (10 MACE steps x 5 SIRT) on a 256x256x64 box, 61 views
+    "c2_sched_256": ("c2", dict(X=256, Y=256, Z=64, n_ell=100), 10, 5),
+}
+N_SAMPLE = 32768
+SAMPLE_SEED = 20261204
+
+
+def projections_hash(pb) -> str:
+    h = hashlib.sha256()
+    for p in pb.ps:
+        h.update(np.ascontiguousarray(p, np.float32).tobytes())
+    return h.hexdigest()
+
+
+def sample_indices(N: int) -> np.ndarray:
+    rng = np.random.default_rng(SAMPLE_SEED)
+    idx = np.unique(rng.integers(0, N, size=N_SAMPLE + N_SAMPLE // 8, dtype=np.int64))
+    return idx[:N_SAMPLE]
+
+
+def problem(mtr, case: str):
+    cfg, over, M, S = CASES[case]
+    pb = mtr.synth.config(cfg, **over)
+    return pb, M, S
+
+
+def make(case: str) -> None:
+    mtr = importlib.import_module("multi-tilt-reconstuction_amd")
+    import oracle as orc  # the CPU checker
+
+    t0 = time.time()
+    pb, M, S = problem(mtr, case)
+    print(f"{case}: generated {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z} K={pb.K} in {time.time() - t0:.1f}s", flush=True)
+    idx = sample_indices(pb.N)
+    out = {"idx": idx, "p_sha256": np.array(projections_hash(pb)),
+           "meta": np.array(json.dumps({"case": case, "config": CASES[case][0], "overrides": CASES[case][1],
+                                        "M": M, "S": S, "lam": pb.lam, "rho": pb.rho}))}
+    for mode, fn, dt in (("ref", orc.mace, np.float32), ("f64", orc.mace_f64, np.float64)):
+        t0 = time.time()
+        w = fn(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, M, S, pb.lam, pb.rho)
+        print(f"{case}: oracle {mode} in {time.time() - t0:.0f}s", flush=True)
+        flat = w.reshape(-1)
+        out[mode] = flat[idx].astype(dt)
+        out[mode + "_planes"] = w.reshape(pb.vol.Z, -1).sum(axis=1, dtype=np.float64)
+        out[mode + "_norm2"] = np.array(float(np.dot(flat.astype(np.float64), flat.astype(np.float64))))
+        del w, flat
+    path = os.path.join(HERE, f"large_{case}.npz")
+    np.savez_compressed(path, **out)
+    print(f"{case}: wrote {path}", flush=True)
+
+
+if __name__ == "__main__":
+    for c in (sys.argv[1:] or list(CASES)):
+        make(c)

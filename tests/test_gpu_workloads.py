@@ -1,0 +1,138 @@
+"""GPU parity of the benchmark workloads in the form they are benchmarked in.
+
+The fast solve (kind-sorted batched launches, one HIP graph replayed per MACE
+step, per-agent and joint tile autotuning) against the CPU oracle's
+BPT-sum initialisation and MACE loop (MultiAxis.cpp:40-71, 163-200 around
+FunctionOnGPU.cu:212-244), in both oracle modes:
+
+  four-axis (0/45/90/135 deg), 61 views   128x128x32, M=2, S=2, oracle run here
+  four-axis, full size 1024x1024x256       M=1, S=1, against tests/golden/large_c3_m1s1.npz
+  dual-axis (0/90 deg) full schedule       256x256x64, M=10, S=5, against large_c2_sched_256.npz
+  three-axis (0/60/120 deg) in FP32, BF16 and FP16 storage against the FP64 oracle
+
+Bars (relative L2 = ||gpu - oracle|| / ||oracle||): FP32 storage <= 1e-5
+against both oracle modes; BF16 <= 2e-2 and FP16 <= 3e-3 against FP64 (the
+measured errors are printed).  The full-size fixtures hold a seeded sample
+of 32768 voxels of the oracle's w-bar, its per-Z-plane sums and its squared
+norm; the relative L2 error is taken over the sample, and the plane sums
+and norm must agree to 1e-5 relative.  The fixtures are written by
+tests/golden/make_large_fixtures.py, which also records a hash of the
+generated projections: a test first checks that it regenerated the same
+inputs.
+"""
+import importlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SOLVE_TOL = 1e-5
+SPLIT_BOUND = {"fp32": 1e-5, "bf16": 2e-2, "fp16": 3e-3}
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def fixtures():
+    sys.path.insert(0, GOLD)
+    return importlib.import_module("make_large_fixtures")
+
+
+def load_ctx(mtr, pb, prec=None):
+    c = mtr.Context(0) if prec is None else mtr.Context(0, prec)
+    c.set_volume(pb.vol)
+    for a in range(pb.K):
+        c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+    return c
+
+
+def reduced(mtr, cfg, X=128, Y=128, Z=32, n_ell=60):
+    return mtr.synth.config(cfg, X=X, Y=Y, Z=Z, n_ell=n_ell)
+
+
+def test_four_axis_fast_solve_matches_both_oracles(mtr, orc):
+    pb = reduced(mtr, "c3")
+    assert pb.K == 4 and pb.prjs[0].AngN == 61
+    M, S = 2, 2
+    say("four-axis 128x128x32, 61 views, M=2 S=2, graph + joint autotune")
+    with load_ctx(mtr, pb) as c:
+        c.use_graphs(True)
+        c.autotune(True)
+        kinds = [c.agent_kernels(a) for a in range(pb.K)]
+        # 0 and 90 deg: row-separable kernels; 45 and 135 deg: column plan
+        assert kinds[0] == kinds[2] == (mtr.FPK_STREAM, mtr.BPK_SEPARABLE)
+        assert kinds[1] == kinds[3] == (mtr.FPK_COLUMN, mtr.BPK_COLUMN)
+        out = c.mace_solve(M, S, pb.lam, pb.rho)
+        again = c.mace_solve(M, S, pb.lam, pb.rho)
+    ref = orc.mace(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, M, S, pb.lam, pb.rho)
+    ref64 = orc.mace_f64(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, M, S, pb.lam, pb.rho)
+    e, e64 = orc.rel_l2(out, ref), orc.rel_l2(out, ref64)
+    say(f"four-axis rel_l2 vs ref oracle {e:.3e}, vs f64 oracle {e64:.3e}")
+    assert e < SOLVE_TOL and e64 < SOLVE_TOL
+    # the second solve replays the cached graph with the tuned tiles
+    assert np.array_equal(out, again)
+
+
+def _against_fixture(mtr, orc, case):
+    fx = fixtures()
+    path = os.path.join(GOLD, f"large_{case}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{os.path.basename(path)} not generated (tests/golden/make_large_fixtures.py {case})")
+    g = np.load(path)
+    pb, M, S = fx.problem(mtr, case)
+    assert str(g["p_sha256"]) == fx.projections_hash(pb), "generator output differs from the fixture's inputs"
+    say(f"{case}: {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, K={pb.K}, M={M} S={S}")
+    with load_ctx(mtr, pb) as c:
+        out = c.mace_solve(M, S, pb.lam, pb.rho)
+    flat = out.reshape(-1)
+    idx = g["idx"]
+    planes = out.reshape(pb.vol.Z, -1).sum(axis=1, dtype=np.float64)
+    norm2 = float(np.dot(flat.astype(np.float64), flat.astype(np.float64)))
+    del out
+    res = {}
+    for mode in ("ref", "f64"):
+        e = orc.rel_l2(flat[idx], g[mode])
+        ep = float(np.abs(planes - g[mode + "_planes"]).max() / np.abs(g[mode + "_planes"]).max())
+        en = abs(norm2 - float(g[mode + "_norm2"])) / float(g[mode + "_norm2"])
+        say(f"{case}: vs {mode} oracle: sampled rel_l2 {e:.3e}, plane sums {ep:.3e}, norm2 {en:.3e}")
+        res[mode] = (e, ep, en)
+    for e, ep, en in res.values():
+        assert e < SOLVE_TOL and ep < SOLVE_TOL and en < SOLVE_TOL
+
+
+def test_four_axis_full_size_one_step_matches_oracle(mtr, orc):
+    _against_fixture(mtr, orc, "c3_m1s1")
+
+
+def test_dual_axis_full_schedule_matches_oracle(mtr, orc):
+    _against_fixture(mtr, orc, "c2_sched_256")
+
+
+@pytest.fixture(scope="module")
+def three_axis(mtr, orc):
+    pb = reduced(mtr, "c4")
+    assert pb.K == 3 and pb.prjs[0].AngN == 61
+    ref64 = orc.mace_f64(pb.prjs, pb.vol, pb.xcs, pb.ycs, pb.ps, 2, 2, pb.lam, pb.rho)
+    return pb, ref64
+
+
+@pytest.mark.parametrize("name", ["fp32", "bf16", "fp16"])
+def test_three_axis_split_precision_against_fp64_oracle(mtr, orc, three_axis, name):
+    pb, ref64 = three_axis
+    prec = {"fp32": mtr.PREC_FP32, "bf16": mtr.PREC_BF16, "fp16": mtr.PREC_FP16}[name]
+    with load_ctx(mtr, pb, prec) as c:
+        kinds = [c.agent_kernels(a) for a in range(pb.K)]
+        assert kinds[0] == (mtr.FPK_STREAM, mtr.BPK_SEPARABLE)
+        assert kinds[1] == kinds[2] == (mtr.FPK_COLUMN, mtr.BPK_COLUMN)
+        out = c.mace_solve(2, 2, pb.lam, pb.rho)
+    err = orc.rel_l2(out, ref64)
+    say(f"three-axis 0/60/120 128x128x32, 61 views, M=2 S=2, {name}: rel_l2 vs FP64 oracle = {err:.3e} "
+        f"(bound {SPLIT_BOUND[name]:.0e})")
+    assert np.isfinite(out).all()
+    assert err < SPLIT_BOUND[name]
