@@ -351,6 +351,7 @@ struct Agent {
     ColMode col;               // column plan (oblique in-plane axes)
     bool tuned = false;        // back-projection tiles chosen by the autotuner
     int col_win = 0, col_jit = 0, col_span = 0;  // column FP row window (slots, power of two; 0: whole width)
+    bool col_pfx = false;      // column FP from prefix moments (k_fp_pfx) rather than the segmented scan
     int2 *d_col_xy = nullptr;
     float2 *d_col_w = nullptr;
     int *d_strip = nullptr;
@@ -564,8 +565,13 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
         Timed t(ctx, MTR_K_NORMALISER);
         if (ag.col.on) {
             const AgentDev d = make_dev(ctx, ag);
-            HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), fp_col_slots(ag), true,
-                                  ctx->stream));
+            if (ag.col_pfx)
+                HIP_TRY(launch_fp_pfx(ctx->precision, ctx->d_ags + a, 1, fp_pfx_blocks(d),
+                                      fp_pfx_lds(ag.col_win, ag.prj.X + 2, ctx->vol.Z, true), true, ag.col_win > 0,
+                                      ctx->stream));
+            else
+                HIP_TRY(launch_fp_col(ctx->precision, ctx->d_ags + a, 1, fp_col_blocks(d), fp_col_slots(ag), true,
+                                      ctx->stream));
             HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + a, 1, fp_col_res_blocks(d), 2, ag.d_c,
                                       ctx->stream));
             // and the back-projection normaliser of every voxel
@@ -595,7 +601,8 @@ struct Batch {
 
 int fp_kind_key(const Agent &a)
 {
-    return a.col.on ? 100 + a.col.rw : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g : (a.lanes_k ? 1 : 0));
+    return a.col.on ? 100 + a.col.rw + (a.col_pfx ? 1000 : 0) + (a.col_win > 0 ? 2000 : 0)
+                    : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g : (a.lanes_k ? 1 : 0));
 }
 int bp_kind_key(const Agent &a)
 {
@@ -611,15 +618,19 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
         const Agent &a0 = *B.e[s].ag;
         int e = s + 1;
         while (e < end && fp_kind_key(*B.e[e].ag) == fp_kind_key(a0)) e++;
-        int mb = 0, mr = 0, nu = 0;
+        int mb = 0, mr = 0, nu = 0, lds = 0;
+        bool pfx = true;
+        for (int q = s; q < e; q++) pfx = pfx && B.e[q].ag->col_pfx;
+        const bool win = a0.col_win > 0;  // the kind key keeps windowed and whole-row agents apart
         for (int q = s; q < e; q++) {
             Agent &ag = *B.e[q].ag;
             const AgentDev d = make_dev(B.e[q].owner, ag);
             if (a0.col.on) {
                 ag.npartial = fp_col_res_blocks(d);
-                mb = std::max(mb, fp_col_blocks(d));
+                mb = std::max(mb, pfx ? fp_pfx_blocks(d) : fp_col_blocks(d));
                 mr = std::max(mr, ag.npartial);
                 nu = std::max(nu, fp_col_slots(ag));
+                if (pfx) lds = std::max(lds, fp_pfx_lds(ag.col_win, ag.prj.X + 2, d.Z, false));
             } else {
                 ag.npartial = ag.fp_nc ? fp_stream_blocks(d, ag.fp_g) : ag.nfp;
                 mb = std::max(mb, ag.npartial);
@@ -629,7 +640,9 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
         if (a0.col.on) {
             {
                 Timed tk(eng, MTR_K_FP_COLUMN);
-                HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
+                if (pfx)
+                    HIP_TRY(launch_fp_pfx(eng->precision, B.d_tab + s, e - s, mb, lds, false, win, eng->stream));
+                else HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
             }
             Timed tk(eng, MTR_K_FP_COLUMN_RES);
             HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
@@ -973,12 +986,14 @@ int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda
 // extra waves gain: 4.29 -> 4.43 ms at L = 770), or when no window smaller
 // than L suffices; rev: walk each strip backwards.
 void col_window(const std::vector<int2> &xy, const std::vector<int> &cnt, const double *xc, int A,
-                const mtr_volume_size &vol, int nu, int *win, int *jit, int *span, bool *rev)
+                const mtr_volume_size &vol, int nu, bool pfx, int *win, int *jit, int *span, bool *rev)
 {
     const int L = nu + 2, ns = (int)cnt.size() - 1;
     *win = 0;
     *rev = false;
-    if ((160 << 10) / (fp_col_waves() * L * (int)sizeof(float2)) >= 6) return;
+    // the prefix-moment kernel keeps eight waves' rows beside the staged
+    // moments, and always takes a window smaller than whole rows
+    if (!pfx && (160 << 10) / (fp_col_waves() * L * (int)sizeof(float2)) >= 6) return;
     const int zs = vol.Zstart, nz = vol.Z;
     // views [v0, v1): (largest drop, largest span)
     auto pass = [&](bool back, int v0, int v1) {
@@ -1150,11 +1165,15 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                 wts[at] = make_float2((float)(1.0 - dvs[c]), (float)dvs[c]);
             }
         bool rev = false;
-        col_window(xy, cnt, xc, A, vb, prj->X, &ag.col_win, &ag.col_jit, &ag.col_span, &rev);
+        const char *pf = std::getenv("MTR_FP_COL_PFX");
+        ag.col_pfx = !(pf && std::atoi(pf) == 0);
+        col_window(xy, cnt, xc, A, vb, prj->X, ag.col_pfx, &ag.col_win, &ag.col_jit, &ag.col_span, &rev);
         if (const char *w = std::getenv("MTR_FP_COL_WINDOW"); w && std::atoi(w) == 0) {
             ag.col_win = 0;
             rev = false;
         }
+        // the moments of one column and the rows must fit one block's LDS
+        if (ag.col_pfx && fp_pfx_lds(ag.col_win, prj->X + 2, vb.Z, false) == 0) ag.col_pfx = false;
         if (rev)
             for (int q = 0; q < nstrip; q++) {
                 std::reverse(xy.begin() + cnt[q], xy.begin() + cnt[q + 1]);

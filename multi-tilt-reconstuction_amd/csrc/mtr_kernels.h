@@ -28,6 +28,15 @@ int fp_col_blocks(const AgentDev &ag);
 int fp_col_max_nu();
 hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_blocks, int slots, bool unit,
                          hipStream_t st);
+// The same forward projection from per-column prefix moments (8 views per
+// block sharing each column's moments): lds = fp_pfx_lds(col_win, nu + 2, Z, unit)
+// bytes of the batch's largest agent; 0 when a column's moments and the
+// rows do not fit (use launch_fp_col then).
+int fp_pfx_blocks(const AgentDev &ag);
+int fp_pfx_lds(int col_win, int L, int nz, bool unit);
+// win: every agent of the batch keeps a row window (col_win > 0), or none does.
+hipError_t launch_fp_pfx(int prec, const AgentDev *d_ags, int nag, int max_blocks, int lds, bool unit, bool win,
+                         hipStream_t st);
 int fp_col_res_blocks(const AgentDev &ag);
 hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int blocks, int mode, float *c_out,
                              hipStream_t st);

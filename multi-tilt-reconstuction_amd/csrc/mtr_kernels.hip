@@ -1519,6 +1519,332 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
     rw.flush(L, lane);  // the rest of the rows (all of them without a window)
 }
 
+// ------------------------------------------------------------------------
+// Column-plan forward projection by prefix moments.
+//
+// Along a column (X, Y) the planes z = 0 .. Z-1 of the box (absolute Zs + z)
+// project in view a to u(z) = ub + c3 z (ub = fma(c3, Zs, bxy) - 1, bxy the
+// reference's chain), all onto rows k0, k0 + 1.  Detector column i collects
+//     t(i) = sum_z x(z) max(0, 1 - |u(z) - i|),
+// the bilinear tap weights of every plane that reaches it.  With the
+// column's prefix sums P0(b) = sum_{z<b} x(z) and P1(b) = sum_{z<b} z x(z),
+// the ramp moment X2(s) = sum_{z<s} x(z) (s - z) = s P0(ceil s) - P1(ceil s)
+// is exact for any real s (clamping ceil s to [0, Z]), and the tent sum is
+// its second difference at the plane positions where u crosses i - 1, i and
+// i + 1:
+//     t(i) = |c3| (X2(s_{i+1}) - 2 X2(s_i) + X2(s_{i-1})),  s_m = (m - ub) / c3,
+// for either sign of c3 (the ramp is continuous, so a plane at a crossing
+// adds the same whichever side it is counted on).  A column then costs one
+// pass per 62 detector columns it reaches (|c3| Z + 2 of them) instead of a
+// tap and a segmented reduction per plane: lane l forms X2 at s_{p-1+l},
+// lanes l - 1 and l - 2 hand theirs over (DPP shifts), and lanes 2..63 own
+// pixels p .. p + 61.  The moments are formed in FP64 (the second
+// difference cancels about 16 of its 53 bits) once per column and block,
+// shared by the block's eight views (one per wave); t is rounded once to
+// FP32 and added, weighted (1 - dv, dv), into the wave's two detector rows
+// in the strip's fixed column order, with the row window of ColRows.  An
+// untilted view (c3 = 0) puts every plane at ub: (1 - du) P0(Z) and
+// du P0(Z) at floor(ub) and floor(ub) + 1.  UNIT (the normaliser, x = 1)
+// takes the moments in closed form, P0 = b, P1 = b (b - 1) / 2.
+constexpr int kPfxWaves = 8;
+constexpr int kPfxMaxCols = 8;
+constexpr int kPfxLds = 65536;
+
+// 64-bit DPP moves (both halves with the same pattern): zero-fill and
+// row-masked forms.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_z<CTRL>((int)(unsigned)b), hi = (unsigned)dpp_z<CTRL>((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d_rows(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_old<CTRL, ROWS>(0, (int)(unsigned)b);
+    const unsigned hi = (unsigned)dpp_old<CTRL, ROWS>(0, (int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Inclusive sum over the wave: Hillis-Steele inside rows of 16 lanes, then
+// the row carries (row_bcast 15 into rows 1 and 3, row_bcast 31 into 2, 3).
+__device__ __forceinline__ double wave_scan_d(double v)
+{
+    v += dpp_d<0x111>(v);
+    v += dpp_d<0x112>(v);
+    v += dpp_d<0x114>(v);
+    v += dpp_d<0x118>(v);
+    v += dpp_d_rows<0x142, 0xa>(v);
+    v += dpp_d_rows<0x143, 0xc>(v);
+    return v;
+}
+
+// Row slots per wave: the window, or whole rows with room past their end
+// for a pass that runs beyond the last pixel (its slots there only ever
+// receive zeros and are never written out).
+__host__ __device__ inline int pfx_slots(int col_win, int L) { return col_win > 0 ? col_win : ((L + 1) & ~1) + 128; }
+
+// Columns whose moments one block stages at a time (0: does not fit).
+__host__ __device__ inline int pfx_cols(int nslot, int nz)
+{
+    const int left = kPfxLds - kPfxWaves * nslot * (int)sizeof(float2);
+    const int per = (nz + 1) * 16;
+    if (left < per) return 0;
+    return left / per < kPfxMaxCols ? left / per : kPfxMaxCols;
+}
+__host__ __device__ inline int pfx_lds_bytes(int nslot, int nz, bool unit)
+{
+    return kPfxWaves * nslot * (int)sizeof(float2) + (unit ? 0 : pfx_cols(nslot, nz) * (nz + 1) * 16);
+}
+
+// The first 256 planes of a column, 4 per lane, loaded ahead of its
+// moments (the loads of the next round overlap this round's evaluation).
+template <class T>
+__device__ __forceinline__ void pfx_load(const __attribute__((address_space(1))) T *xcol, int zb, int nz, int lane,
+                                         float (&v)[4])
+{
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int z = zb + 4 * lane + r;
+        v[r] = z < nz ? ldg<T>(xcol, (uint32_t)z) : 0.0f;
+    }
+}
+
+// Prefix moments of one column (the calling wave): m[b] = (P0(b), P1(b))
+// for b = 0 .. Z, 4 planes per lane and 256 per pass; `first` holds the
+// first pass's values.
+template <class T>
+__device__ __forceinline__ void pfx_column(const __attribute__((address_space(1))) T *xcol, int nz, double2 *m,
+                                           int lane, const float (&first)[4])
+{
+    double car0 = 0.0, car1 = 0.0;
+    float v[4] = {first[0], first[1], first[2], first[3]};
+    for (int zb = 0; zb < nz; zb += 256) {
+        if (zb > 0) pfx_load<T>(xcol, zb, nz, lane, v);
+        double q0[4], q1[4], t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const double e = (double)v[r];
+            q0[r] = t0;
+            q1[r] = t1;
+            t0 += e;
+            t1 = fma((double)(zb + 4 * lane + r), e, t1);
+        }
+        const double i0 = wave_scan_d(t0), i1 = wave_scan_d(t1);
+        const double x0 = dpp_d<0x138>(i0) + car0, x1 = dpp_d<0x138>(i1) + car1;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int z = zb + 4 * lane + r;
+            if (z < nz) m[z] = make_double2(x0 + q0[r], x1 + q1[r]);
+        }
+        car0 += readlane_d(i0, 63);
+        car1 += readlane_d(i1, 63);
+    }
+    if (lane == 0) m[nz] = make_double2(car0, car1);
+}
+
+// Column table entries through the scalar cache (uniform index).
+__device__ __forceinline__ int2 ucol_xy(const int2 *tab, int j)
+{
+    const __attribute__((address_space(4))) int *t = (const __attribute__((address_space(4))) int *)tab;
+    const int q = 2 * __builtin_amdgcn_readfirstlane(j);
+    return make_int2(t[q], t[q + 1]);
+}
+__device__ __forceinline__ float2 ucol_w(const float2 *tab, int j)
+{
+    const __attribute__((address_space(4))) float *t = (const __attribute__((address_space(4))) float *)tab;
+    const int q = 2 * __builtin_amdgcn_readfirstlane(j);
+    return make_float2(t[q], t[q + 1]);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// One column in one view (c3 != 0), R = 1 or 2 boundaries per lane: a
+// pass covers boundaries m = p - 1 + R lane + r and pixels i = m (lane 0's
+// first and lane 63's last pixel lack a neighbour and add zero), 64 R - 2
+// pixels.  Every pixel slot of the pass is read and written back; pixels
+// outside the column's on-detector range [ilo, ihi] add exactly zero.
+// 64 R <= window slots, so no two pixels of a pass share a slot.  With two
+// per lane the passes start at even pixels and a lane's two slots are one
+// aligned 16-byte row access (consecutive lanes on consecutive addresses).
+template <int R, bool UNIT, bool WIN>
+__device__ __forceinline__ void pfx_eval(ColRows &rw, const double2 *m, int nz, double ub, int ilo, int ihi,
+                                         double inv, float wa, float wb, int lane)
+{
+    constexpr int PP = 64 * R - 2;
+    const double sb = -ub * inv, lanefR = (double)(R * lane);
+    for (int p = R == 2 ? (ilo & ~1) : ilo; p <= ihi; p += PP) {
+        const double s0 = fma(lanefR + (double)(p - 1), inv, sb);
+        double X2[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double sg = r == 0 ? s0 : s0 + inv;
+            const int b = (int)__builtin_amdgcn_fmed3f(ceilf((float)sg), 0.0f, (float)nz);
+            double2 mm;
+            if constexpr (UNIT) {
+                mm = make_double2((double)b, 0.5 * (double)b * (double)(b - 1));
+            } else {
+                mm = m[b];
+            }
+            X2[r] = fma(sg, mm.x, -mm.y);
+        }
+        const double left = dpp_d<0x138>(X2[R - 1]);  // lane - 1's last (wave_shr 1)
+        const double right = dpp_d<0x130>(X2[0]);     // lane + 1's first (wave_shl 1)
+        const int sl0 = p + R * lane;
+        float t[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const double xm = r == 0 ? left : X2[r - 1];
+            const double xp = r == R - 1 ? right : X2[r + 1];
+            t[r] = (float)(fma(-2.0, X2[r], xm) + xp);
+            if ((r == 0 && lane == 0) || (r == R - 1 && lane == 63) ||
+                (unsigned)(sl0 + r - 1 - ilo) > (unsigned)(ihi - ilo))
+                t[r] = 0.0f;
+        }
+        const int g = WIN ? (sl0 & rw.msk) : sl0;
+        if constexpr (R == 1) {
+            const float2 rv = rw.rw[g];
+            rw.rw[g] = make_float2(fmaf(wa, t[0], rv.x), fmaf(wb, t[0], rv.y));
+        } else {
+            float4 *q = reinterpret_cast<float4 *>(rw.rw + g);
+            const float4 rv = *q;
+            *q = make_float4(fmaf(wa, t[0], rv.x), fmaf(wb, t[0], rv.y), fmaf(wa, t[1], rv.z), fmaf(wb, t[1], rv.w));
+        }
+    }
+}
+
+template <class T, bool UNIT, bool WIN>
+__global__ __launch_bounds__(64 * kPfxWaves) void k_fp_pfx(const AgentDev *__restrict__ ags)
+{
+    extern __shared__ float4 pfx_raw[];
+    const AgentDev &ag = ags[blockIdx.y];
+    const int nvg = (ag.A + kPfxWaves - 1) / kPfxWaves;
+    // XCD-aware order as in k_fp_col: strips dealt to the XCDs in turn, the
+    // view groups of one strip back to back on one XCD
+    const int spx = (ag.nstrip + 7) / 8;
+    const int idx = blockIdx.x >> 3, sx = idx / nvg;
+    if (sx >= spx) return;
+    const int s = sx * 8 + (blockIdx.x & 7);
+    if (s >= ag.nstrip) return;
+    const int lane = threadIdx.x & 63, wid = wave_id();
+    const int a = (idx - sx * nvg) * kPfxWaves + wid;
+    const bool view = a < ag.A;
+    if (UNIT && !view) return;  // no block-wide barrier without staged moments
+    const int nu = ag.nu, nv = ag.nv, L = nu + 2, nz = ag.Z, zs = ag.Zs;
+    const int nslot = pfx_slots(WIN ? ag.col_win : 0, L);
+    const int ncol = UNIT ? kPfxMaxCols : pfx_cols(nslot, nz);
+    float2 *rows = reinterpret_cast<float2 *>(pfx_raw);
+    double2 *mom = reinterpret_cast<double2 *>(rows + kPfxWaves * nslot);
+    const int k0 = s - 1;
+    ColRows rw;
+    rw.rw = rows + wid * nslot;
+    rw.win = WIN;
+    rw.msk = WIN ? ag.col_win - 1 : 0x7fffffff;
+    rw.L = L;
+    rw.nu = nu;
+    rw.jit = ag.col_jit;
+    rw.span = ag.col_span;
+    rw.lo = 0;
+    const int64_t row = ((int64_t)(view ? a : 0) * nv + k0) * nu;
+    rw.pm = view && k0 >= 0 ? ag.pm + row : nullptr;
+    rw.pn = view && k0 + 1 < nv ? ag.pn + row + nu : nullptr;
+    for (int q = lane; q < nslot; q += 64) rw.rw[q] = make_float2(0.0f, 0.0f);
+    wave_lds_sync();
+    double uc[4];
+    load_ucoef(ag.coef, __builtin_amdgcn_readfirstlane(view ? a : 0), uc);
+    const double c0 = uc[0], c1 = uc[1], c2 = uc[2], c3 = uc[3];
+    const double inv = c3 != 0.0 ? 1.0 / c3 : 0.0;
+    const float ac3 = (float)fabs(c3);
+    // boundaries per lane: one where a column reaches at most 62 pixels
+    // (|c3| (Z - 1) + 3 at most), two otherwise (64 R slots in the window)
+    const int npix = (int)(fabs(c3) * (double)(nz - 1)) + 3;
+    const int R = __builtin_amdgcn_readfirstlane(npix <= 62 || (WIN && nslot < 128) ? 1 : 2);
+    const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
+    const auto x = gview(reinterpret_cast<const T *>(ag.x));
+    const int64_t xoff = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy;
+    auto column = [&](int2 xy) { return x + (xoff + (int64_t)xy.x * ag.sx + (int64_t)xy.y * ag.sy); };
+    const int zp = nz + 1;
+    // lane q holds the table entry of the round's column q: loaded a round
+    // ahead, with the first 256 planes of the column the wave stages next
+    int2 txy = make_int2(0, 0);
+    float2 tw = make_float2(0.0f, 0.0f);
+    float xpre[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (lane < min(ncol, j1 - j0)) {
+        txy = ag.col_xy[j0 + lane];
+        tw = ag.col_w[j0 + lane];
+    }
+    if (!UNIT && wid < min(ncol, j1 - j0)) pfx_load<T>(column(ucol_xy(ag.col_xy, j0 + wid)), 0, nz, lane, xpre);
+    for (int jr = j0; jr < j1; jr += ncol) {
+        const int nc = min(ncol, j1 - jr), jn = jr + ncol, ncn = max(min(ncol, j1 - jn), 0);
+        if constexpr (!UNIT) {
+            if (wid < nc) pfx_column<T>(column(ucol_xy(ag.col_xy, jr + wid)), nz, mom + wid * zp, lane, xpre);
+            __syncthreads();
+        }
+        // per-column constants of this view, lane q for column jr + q, with
+        // the reference's chain (and the host's, for the window bounds)
+        const double bxy = fma(c2, (double)txy.y, fma(c1, (double)txy.x, c0));
+        const double ubq = fma(c3, (double)zs, bxy) - 1.0;
+        const double ueq = fma(c3, (double)(zs + nz - 1), bxy) - 1.0;
+        const int flo = (int)floor(fmin(ubq, ueq)), fhi = (int)floor(fmax(ubq, ueq));
+        const float waq = tw.x * ac3, wbq = tw.y * ac3, wzq = tw.x, wyq = tw.y;
+        // the next round's table entries and planes load during this round
+        if (lane < ncn) {
+            txy = ag.col_xy[jn + lane];
+            tw = ag.col_w[jn + lane];
+        }
+        if (!UNIT && wid < ncn) pfx_load<T>(column(ucol_xy(ag.col_xy, jn + wid)), 0, nz, lane, xpre);
+        if (view) {
+            for (int q = 0; q < nc; q++) {
+                const double ub = readlane_d(ubq, q);
+                const int lo = __builtin_amdgcn_readlane(flo, q), hi = __builtin_amdgcn_readlane(fhi, q);
+                const double2 *m = mom + q * zp;
+                if constexpr (WIN) {
+                    // slots below the column's first slot less the jitter bound
+                    // are final once its slots would leave the window
+                    const int sa = lo + 1;
+                    if (sa + rw.span >= rw.lo + rw.msk + 1) {
+                        rw.flush(min(sa - rw.jit, L), lane);
+                        wave_lds_sync();
+                    }
+                }
+                if (c3 == 0.0) {
+                    if (lane < 2) {
+                        const double fl = floor(ub), du = ub - fl;
+                        const int i = (int)fl + lane;
+                        const double S = UNIT ? (double)nz : m[nz].x;
+                        const float t = (float)((lane == 0 ? 1.0 - du : du) * S);
+                        if (i >= 0 && i < nu) {
+                            float2 &c = rw.at(i + 1);
+                            c.x = fmaf(readlane_f(wzq, q), t, c.x);
+                            c.y = fmaf(readlane_f(wyq, q), t, c.y);
+                        }
+                    }
+                } else {
+                    const int ilo = max(lo, 0), ihi = min(hi + 1, nu - 1);
+                    const float wa = readlane_f(waq, q), wb = readlane_f(wbq, q);
+                    if (R == 1) pfx_eval<1, UNIT, WIN>(rw, m, nz, ub, ilo, ihi, inv, wa, wb, lane);
+                    else pfx_eval<2, UNIT, WIN>(rw, m, nz, ub, ilo, ihi, inv, wa, wb, lane);
+                }
+                wave_lds_sync();
+            }
+        }
+        if constexpr (!UNIT) __syncthreads();
+    }
+    wave_lds_sync();
+    if (view) rw.flush(L, lane);
+}
+
 // s = pm + pn; MODE 0: residual r = p - (c != 0 ? s/c : s) with FP64
 // per-block sums of r^2; MODE 2: the normaliser c = s into c_out.
 template <class T, int MODE>
@@ -2491,6 +2817,41 @@ hipError_t launch_fp_col(int prec, const AgentDev *d_ags, int nag, int max_block
     if (prec == MTR_PREC_BF16) return launch_fp_col_t<__hip_bfloat16>(d_ags, nag, max_blocks, slots, unit, st);
     if (prec == MTR_PREC_FP16) return launch_fp_col_t<__half>(d_ags, nag, max_blocks, slots, unit, st);
     return launch_fp_col_t<float>(d_ags, nag, max_blocks, slots, unit, st);
+}
+
+int fp_pfx_blocks(const AgentDev &ag)
+{
+    return 8 * ((ag.nstrip + 7) / 8) * ((ag.A + kPfxWaves - 1) / kPfxWaves);
+}
+
+int fp_pfx_lds(int col_win, int L, int nz, bool unit)
+{
+    const int slots = pfx_slots(col_win, L);
+    return pfx_cols(slots, nz) > 0 || unit ? pfx_lds_bytes(slots, nz, unit) : 0;
+}
+
+template <class T>
+static hipError_t launch_fp_pfx_t(const AgentDev *d_ags, int nag, int max_blocks, int lds, bool unit, bool win,
+                                  hipStream_t st)
+{
+    dim3 grid(max_blocks, nag), block(64 * kPfxWaves);
+    if (win) {
+        if (unit) hipLaunchKernelGGL((k_fp_pfx<T, true, true>), grid, block, lds, st, d_ags);
+        else hipLaunchKernelGGL((k_fp_pfx<T, false, true>), grid, block, lds, st, d_ags);
+    } else {
+        if (unit) hipLaunchKernelGGL((k_fp_pfx<T, true, false>), grid, block, lds, st, d_ags);
+        else hipLaunchKernelGGL((k_fp_pfx<T, false, false>), grid, block, lds, st, d_ags);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_pfx(int prec, const AgentDev *d_ags, int nag, int max_blocks, int lds, bool unit, bool win,
+                         hipStream_t st)
+{
+    if (lds <= 0 || lds > kPfxLds) return hipErrorInvalidValue;
+    if (prec == MTR_PREC_BF16) return launch_fp_pfx_t<__hip_bfloat16>(d_ags, nag, max_blocks, lds, unit, win, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_pfx_t<__half>(d_ags, nag, max_blocks, lds, unit, win, st);
+    return launch_fp_pfx_t<float>(d_ags, nag, max_blocks, lds, unit, win, st);
 }
 
 int fp_col_res_blocks(const AgentDev &ag)
