@@ -357,6 +357,9 @@ struct Agent {
     int *d_strip = nullptr;
     float *d_pm = nullptr, *d_pn = nullptr;
     float *d_bpc = nullptr;    // column plan: BP normaliser per voxel
+    bool col_cs = false;       // column plan: back-projection on strip tiles (k_bp_cs)
+    ColGroup *d_col_grp = nullptr;
+    int ngrp = 0;
     float *d_c = nullptr;
     void *d_r = nullptr;
     void *d_x = nullptr;
@@ -515,6 +518,8 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.pm = ag.d_pm;
     d.pn = ag.d_pn;
     d.bpc = ag.d_bpc;
+    d.col_grp = ag.d_col_grp;
+    d.ngrp = ag.ngrp;
     return d;
 }
 
@@ -575,6 +580,9 @@ int ensure_normaliser(mtr_ctx *ctx, int a)
             HIP_TRY(launch_fp_col_res(ctx->precision, ctx->d_ags + a, 1, fp_col_res_blocks(d), 2, ag.d_c,
                                       ctx->stream));
             // and the back-projection normaliser of every voxel
+            if (ag.col_cs)
+                HIP_TRY(launch_bp_cs(ctx->precision, ctx->d_ags + a, 1, bp_cs_blocks(d, 16), 2, 0.0f, ctx->stream));
+            else
             HIP_TRY(launch_bp_col(ctx->precision, ctx->d_ags + a, 1, bp_col_blocks(d, ag.col.rw), 2, 0.0f, ag.col.rw,
                                   ctx->stream));
         } else {
@@ -606,7 +614,8 @@ int fp_kind_key(const Agent &a)
 }
 int bp_kind_key(const Agent &a)
 {
-    return a.col.on ? 100 + a.col.rw : (a.bpm.sep ? 10 + 4 * a.bpm.W + (a.bpm.rw == 32 ? 1 : (a.bpm.rw == 16 ? 2 : 3)) : 0);
+    return a.col.on ? 100 + a.col.rw + (a.col_cs ? 1000 : 0)
+                    : (a.bpm.sep ? 10 + 4 * a.bpm.W + (a.bpm.rw == 32 ? 1 : (a.bpm.rw == 16 ? 2 : 3)) : 0);
 }
 
 // FP + residual of entries [first, first + count).
@@ -672,13 +681,16 @@ int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float
         int mb = 0;
         for (int q = s; q < e; q++) {
             const AgentDev d = make_dev(B.e[q].owner, *B.e[q].ag);
-            mb = std::max(mb, cm.on ? bp_col_blocks(d, cm.rw)
+            mb = std::max(mb, cm.on ? (a0.col_cs ? bp_cs_blocks(d, 16) : bp_col_blocks(d, cm.rw))
                                     : (m.sep ? bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw) : bp_blocks(d, eng->bp_tz)));
         }
         Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP);
         Timed tk(eng, cm.on ? MTR_K_BP_COLUMN : (m.sep ? MTR_K_BP_SEPARABLE : MTR_K_BP_GENERAL), !bpt);
         if (cm.on) {
-            HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, eng->stream));
+            if (a0.col_cs)
+                HIP_TRY(launch_bp_cs(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, eng->stream));
+            else
+                HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, eng->stream));
         } else if (m.sep) {
             HIP_TRY(launch_bp_sep(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, m.W, m.P1, m.P2, m.rw,
                                   eng->stream));
@@ -811,7 +823,7 @@ int apply_tiles(mtr_ctx *ctx, int a, int run, int lanes)
 int tune_agent(mtr_ctx *ctx, int a)
 {
     Agent &ag = ctx->agents[a];
-    if (ag.tuned || !ctx->autotune || !(ag.col.on || ag.bpm.sep)) {
+    if (ag.tuned || !ctx->autotune || !(ag.col.on || ag.bpm.sep) || ag.col_cs) {
         ag.tuned = true;
         return MTR_OK;
     }
@@ -1179,12 +1191,76 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                 std::reverse(xy.begin() + cnt[q], xy.begin() + cnt[q + 1]);
                 std::reverse(wts.begin() + cnt[q], wts.begin() + cnt[q + 1]);
             }
+        // columns whose rows lie off the detector follow the strips: the
+        // forward projection never reads them, the strip-tiled
+        // back-projection gives them C = 0 (x = 0 at the BPT)
+        xy.resize(cnt[nstrip]);
+        wts.resize(cnt[nstrip]);
+        for (int Y = vb.Ystart; Y < vb.Yend; Y++)
+            for (int X = vb.Xstart; X < vb.Xend; X++)
+                if (key[(size_t)(Y - vb.Ystart) * vb.X + (X - vb.Xstart)] < 0) {
+                    xy.push_back(make_int2(X, Y));
+                    wts.push_back(make_float2(0.0f, 0.0f));
+                }
+        const int ncols = (int)xy.size();  // strip columns, then the off-detector ones
+        if (xy.empty()) {
+            xy.push_back(make_int2(0, 0));
+            wts.push_back(make_float2(0.0f, 0.0f));
+        }
         ALLOC(ag.d_col_xy, sizeof(int2) * xy.size());
         ALLOC(ag.d_col_w, sizeof(float2) * wts.size());
         ALLOC(ag.d_strip, sizeof(int) * cnt.size());
         ALLOC(ag.d_pm, sizeof(float) * P);
         ALLOC(ag.d_pn, sizeof(float) * P);
         ALLOC(ag.d_bpc, sizeof(float) * ctx->N);
+        // strip tiles of the back-projection: 64 consecutive columns of a
+        // strip, usable when every tile's footprint in every view is narrow
+        // enough to stage (the kernel's own FP64 bounds, over any block of
+        // bp_cs_planes() planes)
+        {
+            std::vector<ColGroup> grp;
+            for (int q = 0; q <= nstrip; q++) {
+                // q = nstrip: the columns off the detector's rows (k0 = -2)
+                const int ja = q < nstrip ? cnt[q] : cnt[nstrip], jb = q < nstrip ? cnt[q + 1] : ncols;
+                for (int j = ja; j < jb; j += 64) {
+                    ColGroup g{};
+                    g.j0 = j;
+                    g.n = std::min(64, jb - j);
+                    g.xmin = g.ymin = 0x7fffffff;
+                    g.xmax = g.ymax = -0x7fffffff;
+                    for (int i = j; i < j + g.n; i++) {
+                        g.xmin = std::min(g.xmin, xy[i].x);
+                        g.xmax = std::max(g.xmax, xy[i].x);
+                        g.ymin = std::min(g.ymin, xy[i].y);
+                        g.ymax = std::max(g.ymax, xy[i].y);
+                    }
+                    g.k0 = q < nstrip ? q - 1 : -2;
+                    grp.push_back(g);
+                }
+            }
+            const char *cs = std::getenv("MTR_BP_COL_STRIP");
+            ag.col_cs = !grp.empty() && !(cs && std::atoi(cs) == 0);
+            const double zb1 = (double)(bp_cs_planes() - 1);
+            for (size_t gi = 0; gi < grp.size() && ag.col_cs; gi++) {
+                const ColGroup &g = grp[gi];
+                if (g.k0 < -1) continue;  // no footprint
+                for (int v = 0; v < A && ag.col_cs; v++) {
+                    const double *c = xc + 4 * v;
+                    const double ux = std::fmin(c[1] * g.xmin, c[1] * g.xmax), vx = std::fmax(c[1] * g.xmin, c[1] * g.xmax);
+                    const double uy = std::fmin(c[2] * g.ymin, c[2] * g.ymax), vy = std::fmax(c[2] * g.ymin, c[2] * g.ymax);
+                    const double uz = std::fmin(0.0, c[3] * zb1), vz = std::fmax(0.0, c[3] * zb1);
+                    const int w = (int)(std::floor(vx + vy + vz) - std::floor(ux + uy + uz)) + 3;
+                    if (w > cs_max_width() - 2) ag.col_cs = false;
+                }
+            }
+            if (ag.col_cs) {
+                ag.ngrp = (int)grp.size();
+                ALLOC(ag.d_col_grp, sizeof(ColGroup) * grp.size());
+                if (hipMemcpy(ag.d_col_grp, grp.data(), sizeof(ColGroup) * grp.size(), hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                    return undo(fail(MTR_ERR_HIP, "column group upload failed"));
+            }
+        }
         if (hipMemcpy(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ag.d_col_w, wts.data(), sizeof(float2) * wts.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(ag.d_strip, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -1464,7 +1540,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
-                        (void *)ag.d_bpc})
+                        (void *)ag.d_bpc, (void *)ag.d_col_grp})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,
@@ -1569,7 +1645,8 @@ int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size
     // the column plan, x, w and a canonical copy of x, the column plan's
     // back-projection normaliser and tables, FP64 partial sums
     *bytes = (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj->AngN + 2 * 4 * P + 4 * P + es * P +
-             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp;
+             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp +
+             sizeof(ColGroup) * (cols / 64 + (size_t)prj->Y + 2);
     return MTR_OK;
 }
 

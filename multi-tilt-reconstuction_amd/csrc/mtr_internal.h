@@ -52,6 +52,15 @@ struct BpSepView {
     int k0, kstep;
 };
 
+// Back-projection tile of a column-plan agent: up to 64 consecutive
+// columns of one strip (col_xy[j0 .. j0 + n)), all on detector rows k0 and
+// k0 + 1, with the bounding box of their (X, Y).
+struct ColGroup {
+    int j0, n;
+    int xmin, xmax, ymin, ymax;
+    int k0, pad;
+};
+
 // Device-side description of one agent, passed by value to kernels.
 struct AgentDev {
     int nu, nv, A;
@@ -84,6 +93,8 @@ struct AgentDev {
     int col_span;          // FP row window: a column's last slot past its first
     float *pm, *pn;        // FP partial rows [A][nv][nu]: from strip k and strip k - 1
     float *bpc;            // back-projection normaliser C of every voxel (working layout)
+    const ColGroup *col_grp;  // strip-tiled back-projection: column groups
+    int ngrp;
 };
 
 }  // namespace mtr

@@ -45,6 +45,14 @@ hipError_t launch_fp_col_res(int prec, const AgentDev *d_ags, int nag, int block
 // update, 1: BPT, 2: the geometry-only normaliser C into ag.bpc (needed by
 // modes 0 and 1).
 int bp_col_blocks(const AgentDev &ag, int rw);
+// The same back-projection on strip tiles (ag.col_grp, runs of 16 along
+// Z, bp_cs_planes() planes per block); every group's footprint must span at
+// most cs_max_width() detector columns in every view.
+int bp_cs_blocks(const AgentDev &ag, int rw);
+int bp_cs_planes();
+int cs_max_width();
+hipError_t launch_bp_cs(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda,
+                        hipStream_t st);
 int bp_col_waves();
 int fp_col_waves();
 int col_elems();

@@ -2125,6 +2125,196 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
 }
 
 // ------------------------------------------------------------------------
+// Column-plan back-projection on strip tiles.  A block takes one group of
+// up to 64 consecutive columns of a strip (lanes) and eight runs of RW
+// planes along Z (waves).  Every column of a strip lands on the same two
+// detector rows k0, k0 + 1 in every view, so per view the block stages one
+// (r[i], r[i+1] - r[i]) pair of each row per detector column of the
+// group's footprint (the u range of the group's bounding box over the
+// block's planes, at most kCsMaxW columns, checked on the host) as one
+// float4, double buffered; a lane's taps read consecutive or equal
+// addresses.  The arithmetic per voxel and view is k_bp_col's (u at the
+// enclosing multiple of 32 planes in FP64, FP32 steps, the same tap order
+// and update), so the two kernels give identical sums.  Groups with
+// k0 = -2 hold the columns off the detector's rows: no view reaches them,
+// their C is 0 (x = 0 at the BPT, unchanged by SIRT updates).
+constexpr int kCsWaves = 8;
+constexpr int kCsMaxW = 512;
+
+int cs_max_width() { return kCsMaxW - 2; }
+
+template <class T, int RW, int MODE>
+__global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restrict__ ags, float lambda)
+{
+    constexpr bool BPT = MODE == 1, NORM = MODE == 2;
+    constexpr int kEpiF4 = kCsWaves * 64 * 17 / 4;
+    constexpr int kLds = 2 * kCsMaxW > kEpiF4 ? 2 * kCsMaxW : kEpiF4;
+    __shared__ float4 lds4[kLds];
+    __shared__ int64_t cbase[64];
+    const AgentDev &ag = ags[blockIdx.y];
+    const int ntz = col_runs(ag.Zs, ag.Ze, RW), nzg = (ntz + kCsWaves - 1) / kCsWaves;
+    const int nb = ag.ngrp * nzg, per = (nb + 7) / 8;
+    // XCD-aware order: consecutive groups (which share the strip's rows)
+    // run on the same XCD's L2
+    const int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (blockIdx.x >= (unsigned)(per * 8) || b >= nb) return;
+    const int gi = b / nzg, zg = b - gi * nzg;
+    const __attribute__((address_space(4))) int *gp =
+        (const __attribute__((address_space(4))) int *)(ag.col_grp + gi);
+    const int gj0 = gp[0], gn = gp[1], xmin = gp[2], xmax = gp[3], ymin = gp[4], ymax = gp[5], k0 = gp[6];
+    const int lane = threadIdx.x & 63, wid = wave_id();
+    const int zal = col_run_base(ag.Zs, RW);
+    const int bz = zg * kCsWaves + wid;
+    const int Z0 = zal + bz * RW;
+    const int Zb0 = zal + zg * kCsWaves * RW, Zb1 = Zb0 + kCsWaves * RW - 1;
+    const int Z32 = Z0 >= 0 ? (Z0 / 32) * 32 : -(((-Z0) + 31) / 32) * 32;
+    const int zlo = max(ag.Zs - Z0, 0);
+    const int ew = min(RW, ag.Ze - Z0) - 1;
+    const bool wave_ok = bz < ntz;
+    const int nu = ag.nu, nv = ag.nv;
+    // this lane's column (the group's last one for lanes past its end;
+    // stores are masked)
+    const int j = gj0 + min(lane, gn - 1);
+    const int2 xy = ag.col_xy[j];
+    const float2 cw = ag.col_w[j];
+    const int X = xy.x, Y = xy.y;
+    const float wk0 = (unsigned)k0 < (unsigned)nv ? cw.x : 0.0f;
+    const float wk1 = (unsigned)(k0 + 1) < (unsigned)nv ? cw.y : 0.0f;
+    if (wid == 0) cbase[lane] = (int64_t)(X - ag.Xs) * ag.sx + (int64_t)(Y - ag.Ys) * ag.sy;
+    const auto src = gview(reinterpret_cast<const T *>(BPT ? ag.p : ag.r));
+    const int64_t plane = (int64_t)nu * nv;
+    const bool row0_ok = (unsigned)k0 < (unsigned)nv, row1_ok = (unsigned)(k0 + 1) < (unsigned)nv;
+
+    float S[NORM ? 1 : RW], D[NORM ? RW : 1];
+#pragma unroll
+    for (int t = 0; t < RW; t++) {
+        if constexpr (!NORM) S[t] = 0.0f;
+        if constexpr (NORM) D[t] = 0.0f;
+    }
+    // footprint of view a: detector columns over the group's box and the
+    // block's planes, one column of margin on each side
+    auto window = [&](const double (&uc)[4], int &fi0, int &fi1) {
+        const double ux = fmin(uc[1] * (double)xmin, uc[1] * (double)xmax);
+        const double uy = fmin(uc[2] * (double)ymin, uc[2] * (double)ymax);
+        const double uz = fmin(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
+        const double vx = fmax(uc[1] * (double)xmin, uc[1] * (double)xmax);
+        const double vy = fmax(uc[2] * (double)ymin, uc[2] * (double)ymax);
+        const double vz = fmax(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
+        fi0 = (int)floor(uc[0] + ux + uy + uz - 1.0) - 1;
+        fi1 = (int)floor(uc[0] + vx + vy + vz - 1.0) + 1;
+    };
+    float4 pre;
+    auto fetch = [&](int a, int fi0, int fi1) {
+        const int e = threadIdx.x;
+        const int gi0 = fi0 + e;
+        const bool in = e <= fi1 - fi0;
+        const bool ok0 = in && (unsigned)gi0 < (unsigned)nu, ok1 = in && (unsigned)(gi0 + 1) < (unsigned)nu;
+        const uint32_t vb = (uint32_t)(plane * a);
+        const uint32_t i0 = (uint32_t)min(max(gi0, 0), nu - 1), i1 = (uint32_t)min(max(gi0 + 1, 0), nu - 1);
+        const uint32_t o0 = vb + (uint32_t)min(max(k0, 0), nv - 1) * (uint32_t)nu;
+        const uint32_t o1 = vb + (uint32_t)min(max(k0 + 1, 0), nv - 1) * (uint32_t)nu;
+        const float a0 = ok0 && row0_ok ? ldg<T>(src, o0 + i0) : 0.0f, a1 = ok1 && row0_ok ? ldg<T>(src, o0 + i1) : 0.0f;
+        const float b0 = ok0 && row1_ok ? ldg<T>(src, o1 + i0) : 0.0f, b1 = ok1 && row1_ok ? ldg<T>(src, o1 + i1) : 0.0f;
+        pre = make_float4(a0, a1 - a0, b0, b1 - b0);
+    };
+    int fi0, fi1;
+    double ucn[4];
+    load_ucoef(ag.coef, 0, ucn);
+    window(ucn, fi0, fi1);
+    if constexpr (!NORM) {
+        fetch(0, fi0, fi1);
+        lds4[threadIdx.x] = pre;
+    }
+    __syncthreads();
+    const int nview = k0 < -1 ? 0 : ag.A;  // block-uniform
+    for (int a = 0; a < nview; a++) {
+        const int cfi0 = fi0, cfi1 = fi1;
+        const double uc[4] = {ucn[0], ucn[1], ucn[2], ucn[3]};
+        if (a + 1 < nview) {
+            load_ucoef(ag.coef, a + 1, ucn);
+            window(ucn, fi0, fi1);
+            if constexpr (!NORM) fetch(a + 1, fi0, fi1);
+        }
+        const double ub = fma(uc[3], (double)Z32, fma(uc[2], (double)Y, fma(uc[1], (double)X, uc[0]))) - 1.0;
+        const double fb = floor(ub);
+        const float f0 = (float)(ub - fb), step = (float)uc[3];
+        const int ib = (int)fb - cfi0;
+        const float4 *stg = lds4 + (a & 1) * kCsMaxW;
+        const int lmax = min(cfi1 - cfi0, kCsMaxW - 1);
+        auto taps = [&](auto off_c) {
+            constexpr int OFF = decltype(off_c)::value;
+            if constexpr (!NORM) {
+#pragma unroll
+                for (int t = 0; t < RW; t++) {
+                    const float g = fmaf((float)(OFF + t), step, f0);
+                    const float fg = floorf(g);
+                    const float du = g - fg;
+                    const int li = min(max(ib + (int)fg, 0), lmax);
+                    const float4 q = stg[li];
+                    S[t] = fmaf(wk0, fmaf(du, q.y, q.x), S[t]);
+                    S[t] = fmaf(wk1, fmaf(du, q.w, q.z), S[t]);
+                    if ((t & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+                }
+            } else if (!(cfi0 >= 0 && cfi1 + 1 <= nu - 1)) {
+#pragma unroll
+                for (int t = 0; t < RW; t++) {
+                    const float g = fmaf((float)(OFF + t), step, f0);
+                    const float fg = floorf(g);
+                    const float du = g - fg;
+                    const int i0 = ib + cfi0 + (int)fg;
+                    const bool v0 = (unsigned)i0 < (unsigned)nu, v1 = (unsigned)(i0 + 1) < (unsigned)nu;
+                    D[t] += v0 ? (v1 ? 0.0f : du) : (v1 ? 1.0f - du : 1.0f);
+                }
+            }
+        };
+        if (RW == 16 && Z0 != Z32) taps(std::integral_constant<int, 16>{});
+        else taps(std::integral_constant<int, 0>{});
+        if constexpr (!NORM) {
+            if (a + 1 < nview) lds4[((a + 1) & 1) * kCsMaxW + threadIdx.x] = pre;
+            __syncthreads();
+        }
+    }
+    if (NORM || nview == 0) __syncthreads();  // cbase, and the epilogue reuses the staging LDS
+    float *ep = reinterpret_cast<float *>(lds4) + wid * (64 * 17);
+    if constexpr (NORM) {
+        const float kw = wk0 + wk1;
+#pragma unroll
+        for (int t = 0; t < RW; t++) D[t] = kw * ((float)ag.A - D[t]);
+    }
+    T *x = reinterpret_cast<T *>(ag.x);
+    const int64_t zb = (int64_t)(Z0 - ag.Zs) * ag.sz;
+    constexpr int CH = RW < 16 ? RW : 16, LPI = 64 / CH;
+    static_assert(RW % CH == 0 && 64 % CH == 0, "run length must divide into chunks");
+#pragma unroll
+    for (int h = 0; h < RW / CH; h++) {
+#pragma unroll
+        for (int t = 0; t < CH; t++) {
+            if constexpr (NORM) ep[lane * 17 + t] = D[CH * h + t];
+            else ep[lane * 17 + t] = S[CH * h + t];
+        }
+        wave_lds_sync();
+        const int tt = lane % CH, tw = CH * h + tt;
+#pragma unroll 4
+        for (int i = 0; i < CH; i++) {
+            const int line = i * LPI + lane / CH;
+            const float qv = ep[line * 17 + tt];
+            if (wave_ok && tw >= zlo && tw <= ew && line < gn) {
+                const int64_t jv = cbase[line] + zb + tw * ag.sz;
+                if constexpr (NORM) {
+                    ag.bpc[jv] = qv;
+                } else {
+                    const float C = ag.bpc[jv];
+                    const float q = C != 0.0f ? qv / C : 0.0f;
+                    if (BPT) stf<T>(x, jv, q);
+                    else if (C != 0.0f) stf<T>(x, jv, fmaf(q, lambda, ldf<T>(x, jv)));
+                }
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+// ------------------------------------------------------------------------
 // Fused consensus step over all K agents of a solve (element-wise, HBM
 // bound):  z_a = 2 wbar - w_a (the state this step started from);
 // w_a = rho (2 v_a - z_a) + (1 - rho) w_a; wbar' = sum_a w_a / K;
@@ -2883,6 +3073,34 @@ int bp_col_blocks(const AgentDev &ag, int rw)
     const int n1 = ag.col_p1y ? ag.Y : ag.X, n2 = ag.col_p1y ? ag.X : ag.Y;
     const int nb = ((n1 + 63) / 64) * ((n2 + kColBpWaves - 1) / kColBpWaves) * col_runs(ag.Zs, ag.Ze, rw);
     return ((nb + 7) / 8) * 8;
+}
+
+int bp_cs_blocks(const AgentDev &ag, int rw)
+{
+    const int ntz = col_runs(ag.Zs, ag.Ze, rw), nzg = (ntz + kCsWaves - 1) / kCsWaves;
+    const int nb = ag.ngrp * nzg;
+    return ((nb + 7) / 8) * 8;
+}
+
+int bp_cs_planes() { return kCsWaves * 16; }
+
+template <class T>
+static hipError_t launch_bp_cs_t(const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda,
+                                 hipStream_t st)
+{
+    dim3 grid(max_blocks, nag), block(64 * kCsWaves);
+    if (mode == 2) hipLaunchKernelGGL((k_bp_cs<T, 16, 2>), grid, block, 0, st, d_ags, lambda);
+    else if (mode == 1) hipLaunchKernelGGL((k_bp_cs<T, 16, 1>), grid, block, 0, st, d_ags, lambda);
+    else hipLaunchKernelGGL((k_bp_cs<T, 16, 0>), grid, block, 0, st, d_ags, lambda);
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_cs(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda,
+                        hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_bp_cs_t<__hip_bfloat16>(d_ags, nag, max_blocks, mode, lambda, st);
+    if (prec == MTR_PREC_FP16) return launch_bp_cs_t<__half>(d_ags, nag, max_blocks, mode, lambda, st);
+    return launch_bp_cs_t<float>(d_ags, nag, max_blocks, mode, lambda, st);
 }
 
 int bp_col_waves() { return kColBpWaves; }

@@ -193,6 +193,35 @@ def test_agent_kernels_match_oracle(mtr, orc, name, kw):
         assert c.residual_norm2(0) == pytest.approx(float((r * r).sum()), rel=1e-5)
 
 
+CS_GEOMS = [
+    ("col45", dict(phis=[45.0], X=70, Y=66, Z=40, views=13)),
+    ("col135_deep", dict(phis=[135.0], X=40, Y=36, Z=300, views=13, nu=340)),
+    ("col60_small_det", dict(phis=[60.0], X=30, Y=30, Z=20, nu=17, nv=13)),
+    ("col45_straddle", dict(phis=[45.0], X=128, Y=96, Z=8, origin=(0, 0, 12))),
+    # rows of the box's corners off the detector (columns outside every strip)
+    ("col45_corners", dict(phis=[45.0], X=30, Y=26, Z=14, views=9)),
+]
+
+
+@pytest.mark.parametrize("name,kw", CS_GEOMS, ids=[g[0] for g in CS_GEOMS])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp32", "bf16"])
+def test_strip_tiled_back_projection_equals_rectangular_tiles(mtr, name, kw, prec, monkeypatch):
+    # the column plan's back-projection on strip tiles (64 columns of one
+    # strip per block) and on 64 x 8 column tiles: same per-voxel
+    # arithmetic and order, so BPT, normaliser and SIRT volumes agree bit
+    # for bit
+    pb = problem(mtr, **kw)
+    outs = []
+    for strip in ("1", "0"):
+        monkeypatch.setenv("MTR_BP_COL_STRIP", strip)
+        say("strip tiles", name, prec, strip)
+        with load_ctx(mtr, pb, prec) as c:
+            assert c.agent_kernels(0) == (mtr.FPK_COLUMN, mtr.BPK_COLUMN)
+            outs.append((c.bpt(0), c.sirt(0, 2, 0.6)))
+    assert np.isfinite(outs[0][0]).all() and np.isfinite(outs[0][1]).all()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 def load_ctx(mtr, pb, precision=0):
     c = mtr.Context(0, precision)
     c.set_volume(pb.vol)
