@@ -49,8 +49,14 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 vector, packed FMA
 FP32_LANE_OPS = 256 * 64 * 2.4e9  # non-packed issue: 39.3 T lane-ops/s (78.6 TFLOP/s of FMA)
 FLOP_PER_VVU = 8                  # 4 weight x value multiply-adds per bilinear 4-tap evaluation
-HIP_KERNEL = {"fp_stream": "k_fp_stream", "fp_column": "k_fp_col", "fp_column_res": "k_fp_col_res",
-              "fp_general": "k_fp", "bp_separable": "k_bp_sep", "bp_column": "k_bp_col", "bp_general": "k_bp"}
+# kernels behind the per-kernel timing classes (the column plan's forward
+# projection from prefix moments and its strip-tiled back-projection unless
+# the environment keeps the earlier kernels)
+HIP_KERNEL = {"fp_stream": "k_fp_stream",
+              "fp_column": "k_fp_col" if os.environ.get("MTR_FP_COL_PFX") == "0" else "k_fp_pfx",
+              "fp_column_res": "k_fp_col_res", "fp_general": "k_fp", "bp_separable": "k_bp_sep",
+              "bp_column": "k_bp_col" if os.environ.get("MTR_BP_COL_STRIP") == "0" else "k_bp_cs",
+              "bp_general": "k_bp"}
 AXES = {1: "single-axis", 2: "dual-axis", 3: "three-axis", 4: "four-axis"}
 
 

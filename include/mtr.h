@@ -210,11 +210,14 @@ int mtr_set_volume(mtr_ctx *ctx, int which, int agent, const float *host);
  * state between runs): save writes the state w_0 .. w_{K-1}, w-bar as
  * float32 volumes after a 64-byte header (magic "MTRCKPT1", K, the box, the
  * caller's count of steps done, the storage precision, a hash of every
- * agent's detector size and coefficients); load checks magic, agent count,
- * box, geometry hash and file length (MTR_ERR_SHAPE), and NaN or Inf
- * (MTR_ERR_NONFINITE) before it replaces the state, then returns the step
- * count.  mtr_mace_continue resumes from a loaded state bit-exactly when
- * the precision is the one the checkpoint was written in. */
+ * agent's detector size and coefficients; the projection values are not
+ * hashed); save writes a temporary file beside the target and renames it
+ * over the target once written and flushed, so an interrupted save leaves
+ * an earlier checkpoint at that path intact; load checks magic, agent
+ * count, box, geometry hash, storage precision and file length
+ * (MTR_ERR_SHAPE), and NaN or Inf (MTR_ERR_NONFINITE) before it replaces the
+ * state, then returns the step count.  mtr_mace_continue resumes from a
+ * loaded state bit-exactly. */
 int mtr_checkpoint_save(mtr_ctx *ctx, const char *path, int steps_done);
 int mtr_checkpoint_load(mtr_ctx *ctx, const char *path, int *steps_done);
 /* Device pointer of w-bar (FP32 storage only; for device-side consumers). */

@@ -17,6 +17,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "mtr.h"
 #include "mtr_kernels.h"
 
@@ -2097,18 +2099,30 @@ int mtr_checkpoint_save(mtr_ctx *ctx, const char *path, int steps_done)
     h.steps_done = steps_done;
     h.precision = ctx->precision;
     h.geometry = geometry_hash(ctx);
-    File out;
-    if (!(out.f = std::fopen(path, "wb"))) return fail(MTR_ERR_ARG, "cannot create %s", path);
-    if (std::fwrite(&h, sizeof h, 1, out.f) != 1) return fail(MTR_ERR_ARG, "write failed: %s", path);
-    std::vector<float> buf((size_t)ctx->N);
-    for (int v = 0; v <= h.K; v++) {
-        MTR_TRY(copy_volume_out(ctx, v < h.K ? ctx->agents[v].d_w : ctx->d_wbar, buf.data()));
-        if (std::fwrite(buf.data(), sizeof(float), buf.size(), out.f) != buf.size())
-            return fail(MTR_ERR_ARG, "write failed: %s", path);
+    // written beside the target and renamed over it once complete and on
+    // disk, so an interrupted save leaves any earlier checkpoint intact
+    const std::string tmp = std::string(path) + ".tmp";
+    int rc = MTR_OK;
+    {
+        File out;
+        if (!(out.f = std::fopen(tmp.c_str(), "wb"))) return fail(MTR_ERR_ARG, "cannot create %s", tmp.c_str());
+        if (std::fwrite(&h, sizeof h, 1, out.f) != 1) rc = fail(MTR_ERR_ARG, "write failed: %s", tmp.c_str());
+        std::vector<float> buf((size_t)ctx->N);
+        for (int v = 0; v <= h.K && rc == MTR_OK; v++) {
+            rc = copy_volume_out(ctx, v < h.K ? ctx->agents[v].d_w : ctx->d_wbar, buf.data());
+            if (rc == MTR_OK && std::fwrite(buf.data(), sizeof(float), buf.size(), out.f) != buf.size())
+                rc = fail(MTR_ERR_ARG, "write failed: %s", tmp.c_str());
+        }
+        if (rc == MTR_OK && (std::fflush(out.f) != 0 || fsync(fileno(out.f)) != 0))
+            rc = fail(MTR_ERR_ARG, "write failed: %s", tmp.c_str());
+        const int cr = std::fclose(out.f);
+        out.f = nullptr;
+        if (rc == MTR_OK && cr != 0) rc = fail(MTR_ERR_ARG, "write failed: %s", tmp.c_str());
     }
-    const int rc = std::fclose(out.f);
-    out.f = nullptr;
-    return rc == 0 ? MTR_OK : fail(MTR_ERR_ARG, "write failed: %s", path);
+    if (rc == MTR_OK && std::rename(tmp.c_str(), path) != 0)
+        rc = fail(MTR_ERR_ARG, "cannot replace %s", path);
+    if (rc != MTR_OK) (void)std::remove(tmp.c_str());
+    return rc;
 }
 
 int mtr_checkpoint_load(mtr_ctx *ctx, const char *path, int *steps_done)
@@ -2128,6 +2142,10 @@ int mtr_checkpoint_load(mtr_ctx *ctx, const char *path, int *steps_done)
     if (h.geometry != geometry_hash(ctx))
         return fail(MTR_ERR_SHAPE, "checkpoint %s was taken with other projection geometry", path);
     if (h.steps_done < 0) return fail(MTR_ERR_SHAPE, "checkpoint %s: bad step count", path);
+    // a state rounded to another storage type would not continue bit-exactly
+    if (h.precision != ctx->precision)
+        return fail(MTR_ERR_SHAPE, "checkpoint %s was written in storage precision %d, the context uses %d", path,
+                    (int)h.precision, ctx->precision);
     // the whole payload must be there before any state is replaced
     const long long need = (long long)sizeof h + (long long)(h.K + 1) * ctx->N * (long long)sizeof(float);
     if (std::fseek(in.f, 0, SEEK_END) != 0 || std::ftell(in.f) != need)

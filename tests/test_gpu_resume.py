@@ -224,3 +224,15 @@ def test_bad_checkpoints_leave_the_state(mtr, tmp_path):
         with pytest.raises(Exception) as ei:
             c.checkpoint_load(str(good))
         assert ei.value.code == -2
+    # another storage precision would not continue bit-exactly
+    with load(mtr, pb, 1) as c:
+        with pytest.raises(Exception) as ei:
+            c.checkpoint_load(str(good))
+        assert ei.value.code == -2
+    # a save replaces the file whole, through a temporary file beside it
+    with load(mtr, pb) as c:
+        c.mace_solve(2, 1, 0.5, 0.5, out=False)
+        c.checkpoint_save(str(good), 2)
+        assert not (tmp_path / "good.ckpt.tmp").exists()
+        assert c.checkpoint_load(str(good)) == 2
+    assert good.stat().st_size == len(raw)
