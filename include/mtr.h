@@ -286,7 +286,8 @@ enum {
     MTR_K_BP_SEPARABLE = 11, /* runs along the plane axis                  */
     MTR_K_BP_COLUMN = 12,    /* runs along Z                               */
     MTR_K_BP_GENERAL = 13,   /* general plan                               */
-    MTR_K_COUNT = 14
+    MTR_K_FP_STREAM_RES = 14,/* Z-segmented streaming FP: sums + residual  */
+    MTR_K_COUNT = 15
 };
 /* Enable (1) or disable (0) per-launch event timing. */
 int mtr_timing_enable(mtr_ctx *ctx, int on);

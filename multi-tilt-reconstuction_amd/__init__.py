@@ -45,8 +45,9 @@ VOL_WORK, VOL_W, VOL_WBAR = 0, 1, 2
 K_FP, K_BP, K_BPT, K_CONSENSUS, K_NORMALISER, K_ELEMENTWISE, K_SPLAT = range(7)
 # per-kernel parts of the FP and BP classes (SIRT launches only)
 K_FP_STREAM, K_FP_COLUMN, K_FP_COLUMN_RES, K_FP_GENERAL, K_BP_SEPARABLE, K_BP_COLUMN, K_BP_GENERAL = range(7, 14)
+K_FP_STREAM_RES = 14  # Z-segmented streaming FP: segment sums + residual
 KERNEL_NAMES = ["fp", "bp", "bpt", "consensus", "normaliser", "elementwise", "splat", "fp_stream", "fp_column",
-                "fp_column_res", "fp_general", "bp_separable", "bp_column", "bp_general"]
+                "fp_column_res", "fp_general", "bp_separable", "bp_column", "bp_general", "fp_stream_res"]
 
 
 def _check(rc: int, what: str) -> None:

@@ -336,6 +336,8 @@ struct Agent {
     BpMode bpm;            // back-projection kernel choice
     int fp_nc = 0;         // > 0: streaming forward projection with this candidate count
     int fp_g = 1;          // streaming forward projection: detector columns per wave (1, 2)
+    int fp_seg = 1, fp_seg_len = 0;  // streaming forward projection: Z segments and their planes
+    float *d_seg = nullptr;          // per-segment partial ray sums [fp_seg][P] (fp_seg > 1)
     int npartial = 0;      // per-block partials written by the last FP launch
     int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
     int64_t st[3] = {0, 0, 0};
@@ -522,6 +524,9 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.bpc = ag.d_bpc;
     d.col_grp = ag.d_col_grp;
     d.ngrp = ag.ngrp;
+    d.seg_part = ag.d_seg;
+    d.nseg = ag.fp_seg;
+    d.seg_len = ag.fp_seg_len;
     return d;
 }
 
@@ -612,7 +617,7 @@ struct Batch {
 int fp_kind_key(const Agent &a)
 {
     return a.col.on ? 100 + a.col.rw + (a.col_pfx ? 1000 : 0) + (a.col_win > 0 ? 2000 : 0)
-                    : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g : (a.lanes_k ? 1 : 0));
+                    : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g + (a.fp_seg > 1 ? 500 : 0) : (a.lanes_k ? 1 : 0));
 }
 int bp_kind_key(const Agent &a)
 {
@@ -643,8 +648,14 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
                 nu = std::max(nu, fp_col_slots(ag));
                 if (pfx) lds = std::max(lds, fp_pfx_lds(ag.col_win, ag.prj.X + 2, d.Z, false));
             } else {
-                ag.npartial = ag.fp_nc ? fp_stream_blocks(d, ag.fp_g) : ag.nfp;
-                mb = std::max(mb, ag.npartial);
+                if (ag.fp_seg > 1) {
+                    ag.npartial = fp_col_res_blocks(d);
+                    mr = std::max(mr, ag.npartial);
+                    mb = std::max(mb, fp_stream_blocks(d, ag.fp_g));
+                } else {
+                    ag.npartial = ag.fp_nc ? fp_stream_blocks(d, ag.fp_g) : ag.nfp;
+                    mb = std::max(mb, ag.npartial);
+                }
             }
         }
         Timed t(eng, MTR_K_FP);
@@ -658,8 +669,15 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
             Timed tk(eng, MTR_K_FP_COLUMN_RES);
             HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
         } else if (a0.fp_nc) {
-            Timed tk(eng, MTR_K_FP_STREAM);
-            HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, eng->stream));
+            {
+                Timed tk(eng, MTR_K_FP_STREAM);
+                HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, eng->stream));
+            }
+            // the kind key keeps segmented and whole-ray agents apart
+            if (a0.fp_seg > 1) {
+                Timed tk(eng, MTR_K_FP_STREAM_RES);
+                HIP_TRY(launch_fp_seg_res(eng->precision, B.d_tab + s, e - s, mr, eng->stream));
+            }
         } else {
             Timed tk(eng, MTR_K_FP_GENERAL);
             HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, eng->stream));
@@ -1099,7 +1117,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
                         (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
-                        (void *)g.d_bpc})
+                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_seg})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -1120,6 +1138,19 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.fp_g = 2;
     if (const char *g = std::getenv("MTR_FP_COLS")) ag.fp_g = std::atoi(g) == 1 ? 1 : 2;
     ag.fast = ag.fp_nc ? sW : 0;
+    // Streaming FP of a slab (plane axis X or Y) in Z segments of 64 planes
+    // when the box has at least two: the rays a block's views share then
+    // fit one XCD's L2.  MTR_FP_STREAM_SEG sets the planes per segment (0:
+    // whole rays).
+    if (ag.fp_nc && sW != 2) {
+        int len = 64;
+        if (const char *g = std::getenv("MTR_FP_STREAM_SEG")) len = std::max(0, std::atoi(g));
+        const int nz = ctx->vol.Zend - ctx->vol.Zstart;
+        if (len > 0 && nz >= 2 * len) {
+            ag.fp_seg = (nz + len - 1) / len;
+            ag.fp_seg_len = len;
+        }
+    }
     if (!ag.fp_nc && !ag.bpm.sep) {
         ag.col = col_mode(xc, yc, A, *prj, 16);
         if (ag.col.on) ag.fast = 2;  // Z-contiguous columns
@@ -1273,6 +1304,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                                                            fp_col_res_blocks(make_dev(ctx, ag))),
                                                   1));
     ALLOC(ag.d_scalar, sizeof(double));
+    if (ag.fp_seg > 1) ALLOC(ag.d_seg, sizeof(float) * P * ag.fp_seg);
     if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
         hipMemcpyAsync(ag.d_plan, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream) !=
@@ -1542,7 +1574,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
-                        (void *)ag.d_bpc, (void *)ag.d_col_grp})
+                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_seg})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,

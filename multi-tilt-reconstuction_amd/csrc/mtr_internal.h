@@ -95,6 +95,11 @@ struct AgentDev {
     float *bpc;            // back-projection normaliser C of every voxel (working layout)
     const ColGroup *col_grp;  // strip-tiled back-projection: column groups
     int ngrp;
+    // Streaming forward projection in Z segments of seg_len planes (nseg >
+    // 1): per-segment partial ray sums [nseg][A][...] (strides pk/pi),
+    // added and turned into the residual by a second kernel.
+    float *seg_part;
+    int nseg, seg_len;
 };
 
 }  // namespace mtr

@@ -13,6 +13,9 @@ int consensus_blocks(int64_t N);
 int fp_stream_blocks(const AgentDev &ag, int cols_per_wave);
 hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_blocks, int nc, int cols_per_wave,
                             hipStream_t st);
+// Agents streaming in Z segments (ag.nseg > 1): the segments' sums and the
+// residual, grid-stride over the pixels (blocks as fp_col_res_blocks).
+hipError_t launch_fp_seg_res(int prec, const AgentDev *d_ags, int nag, int blocks, hipStream_t st);
 // Row-separable back-projection (runs along the plane axis W).
 // rw: voxels per thread along W (8, 16 or 32).
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw);

@@ -424,23 +424,26 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     __shared__ int tap_off[4][TP];
     __shared__ float tap_w[4][G][TP];
     const AgentDev &ag = ags[blockIdx.y];
-    const int nti = (ag.nu + 4 * G - 1) / (4 * G), ntk = (ag.nv + 63) / 64;
+    const int nti = (ag.nu + 4 * G - 1) / (4 * G), ntk = (ag.nv + 63) / 64, nseg = ag.nseg;
     // XCD-aware order: each XCD takes a contiguous range of (plane group,
-    // column group, view) with the view fastest, so the rays in flight on
-    // one XCD are all views of a few adjacent columns of one plane group and
-    // the voxels they share stay in that XCD's L2 across views.
-    const int nbt = nti * ntk * ag.A, per = (nbt + 7) / 8;
+    // Z segment, column group, view) with the view fastest, so the rays in
+    // flight on one XCD are all views of a few adjacent columns of one
+    // plane group, cut to one segment of seg_len planes: the voxels they
+    // share (about 2 MB for 64 planes) stay in that XCD's L2 across views.
+    const int nbt = nti * ntk * nseg * ag.A, per = (nbt + 7) / 8;
     if (blockIdx.x >= (unsigned)(per * 8)) return;  // beyond this agent's partial sums
     int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (b >= nbt) {
         // padding block of the 8-way split: its partial sum is still read
-        if (threadIdx.x == 0) ag.partial[blockIdx.x] = 0.0;
+        if (threadIdx.x == 0 && nseg == 1) ag.partial[blockIdx.x] = 0.0;
         return;
     }
     const int a = b % ag.A;
     b /= ag.A;
     const int bi = b % nti;
-    const int bk = b / nti;
+    b /= nti;
+    const int sg = b % nseg;
+    const int bk = b / nseg;
     const int lane = threadIdx.x & 63, wid = wave_id();
     const int i0 = (bi * 4 + wid) * G;
     const int gn = min(G, ag.nu - i0);  // this wave's columns on the detector (may be <= 0)
@@ -464,29 +467,37 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     if (gn > 0) {
         const int Llo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.L), Lhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.L) - 1;
         const int Mlo = sel3(ag.Xs, ag.Ys, ag.Zs, pl.M), Mhi = sel3(ag.Xe, ag.Ye, ag.Ze, pl.M) - 1;
+        // this block's segment: planes Z in [zs0, zs1] (the whole box when
+        // nseg = 1); whichever of L and M is Z (the host segments only
+        // agents whose plane axis W is X or Y) is cut to it, the voxel
+        // offsets stay relative to the box's corner
+        const int zs0 = nseg > 1 ? ag.Zs + ag.seg_len * sg : ag.Zs;
+        const int zs1 = nseg > 1 ? min(ag.Ze - 1, zs0 + ag.seg_len - 1) : ag.Ze - 1;
+        const int LloS = pl.L == 2 ? max(Llo, zs0) : Llo, LhiS = pl.L == 2 ? min(Lhi, zs1) : Lhi;
+        const int MloS = pl.M == 2 ? max(Mlo, zs0) : Mlo, MhiS = pl.M == 2 ? min(Mhi, zs1) : Mhi;
         double ms[G];  // aW == 0: no plane term
-        // lines crossing the box, united over the wave's columns
-        int l0 = Lhi + 1, l1 = Llo - 1;
+        // lines crossing the segment, united over the wave's columns
+        int l0 = LhiS + 1, l1 = LloS - 1;
 #pragma unroll
         for (int g = 0; g < G; g++) {
             ms[g] = pl.m0 + (i0 + g) * pl.mi;
             if (g >= gn) continue;
-            double llo = Llo, lhi = Lhi;
+            double llo = LloS, lhi = LhiS;
             bool any = true;
             if (pl.ml != 0.0) {
-                double t0 = (Mlo - pl.Em - 1.0 - ms[g]) / pl.ml, t1 = (Mhi + pl.Em + 1.0 - ms[g]) / pl.ml;
+                double t0 = (MloS - pl.Em - 1.0 - ms[g]) / pl.ml, t1 = (MhiS + pl.Em + 1.0 - ms[g]) / pl.ml;
                 llo = fmax(llo, fmin(t0, t1));
                 lhi = fmin(lhi, fmax(t0, t1));
-            } else if (ms[g] < Mlo - pl.Em - 1.0 || ms[g] > Mhi + pl.Em + 1.0) {
+            } else if (ms[g] < MloS - pl.Em - 1.0 || ms[g] > MhiS + pl.Em + 1.0) {
                 any = false;
             }
             if (any && llo <= lhi) {
-                l0 = min(l0, max(Llo, (int)floor(llo)));
-                l1 = max(l1, min(Lhi, (int)ceil(lhi)));
+                l0 = min(l0, max(LloS, (int)floor(llo)));
+                l1 = max(l1, min(LhiS, (int)ceil(lhi)));
             }
         }
         const double aM = sel3(vc.c[1], vc.c[2], vc.c[3], pl.M);
-        const unsigned mspan = (unsigned)(Mhi - Mlo);
+        const unsigned mspan = (unsigned)(MhiS - MloS);
         // scalar base: voxel (Llo, Mlo) of the box's first plane
         const auto xc = uniform_base(x + base0 + (int64_t)Wlo * sW + (int64_t)Llo * sL + (int64_t)Mlo * sM);
         for (int lb = l0; lb <= l1; lb += 64) {
@@ -511,7 +522,7 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
 #pragma unroll
             for (int j = 0; j < NU; j++) {
                 const int m = mb + j;
-                const bool ok = l <= l1 && (unsigned)(m - Mlo) <= mspan;
+                const bool ok = l <= l1 && (unsigned)(m - MloS) <= mspan;
                 bool any = false;
 #pragma unroll
                 for (int g = 0; g < G; g++) {
@@ -587,8 +598,23 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
+    // rows of planes outside the box see no voxel: sum 0 (their lanes
+    // walked plane Wlo in step with the others)
+    if (nseg > 1) {
+        // the segment's partial sums; k_fp_seg_res adds the segments
+        if (k < ag.nv) {
+            const int64_t P = (int64_t)ag.A * ag.nu * ag.nv;
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                if (g >= gn) break;
+                const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)(i0 + g) * ag.pi;
+                ag.seg_part[(int64_t)sg * P + nidx] = row_ok ? acc[g] : 0.0f;
+            }
+        }
+        return;
+    }
     double r2 = 0.0;
-    if (row_ok) {
+    if (k < ag.nv) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
             if (g >= gn) break;
@@ -596,7 +622,7 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
             // consecutive elements of p, c and r
             const int64_t nidx = (int64_t)a * ag.nu * ag.nv + (int64_t)k * ag.pk + (int64_t)(i0 + g) * ag.pi;
             const float cc = ag.c[nidx];
-            float sv = acc[g];
+            float sv = row_ok ? acc[g] : 0.0f;
             if (cc != 0.0f) sv = sv / cc;
             const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), nidx) - sv;
             stf<T>(reinterpret_cast<T *>(ag.r), nidx, rv);
@@ -1874,6 +1900,30 @@ __global__ __launch_bounds__(256) void k_fp_col_res(const AgentDev *__restrict__
     }
 }
 
+// Segmented streaming forward projection: s = the segments' partial sums
+// added in segment order, then the residual r = p - (c != 0 ? s/c : s) and
+// FP64 per-block sums of r^2, as the fused epilogue forms them.
+template <class T>
+__global__ __launch_bounds__(256) void k_fp_seg_res(const AgentDev *__restrict__ ags)
+{
+    const AgentDev &ag = ags[blockIdx.y];
+    const int64_t P = (int64_t)ag.A * ag.nu * ag.nv;
+    double r2 = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < P; n += (int64_t)gridDim.x * 256) {
+        float sv = ag.seg_part[n];
+        for (int q = 1; q < ag.nseg; q++) sv += ag.seg_part[(int64_t)q * P + n];
+        const float cc = ag.c[n];
+        if (cc != 0.0f) sv = sv / cc;
+        const float rv = ldf<T>(reinterpret_cast<const T *>(ag.p), n) - sv;
+        stf<T>(reinterpret_cast<T *>(ag.r), n, rv);
+        const double rs = (double)rnd<T>(rv);
+        r2 += rs * rs;
+    }
+    __shared__ double sh[4];
+    const double t = block_sum<256>(r2, sh);
+    if (threadIdx.x == 0) ag.partial[blockIdx.x] = t;
+}
+
 // Back-projection of column-plan agents.  Block: 64 lanes along P1 (X, or
 // Y when col_p1y) x 8 waves along the other axis; each thread owns a run
 // of RW voxels along Z.  All voxels of a column read rows k0, k0 + 1 with
@@ -2710,7 +2760,22 @@ hipError_t launch_bp(int prec, const AgentDev *d_ags, int nag, int max_blocks, i
 
 int fp_stream_blocks(const AgentDev &ag, int g)
 {
-    return (((ag.nu + 4 * g - 1) / (4 * g)) * ((ag.nv + 63) / 64) * ag.A + 7) / 8 * 8;
+    return (((ag.nu + 4 * g - 1) / (4 * g)) * ((ag.nv + 63) / 64) * ag.nseg * ag.A + 7) / 8 * 8;
+}
+
+template <class T>
+static hipError_t launch_fp_seg_res_t(const AgentDev *d_ags, int nag, int blocks, hipStream_t st)
+{
+    dim3 grid(blocks, nag), block(256);
+    hipLaunchKernelGGL((k_fp_seg_res<T>), grid, block, 0, st, d_ags);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_seg_res(int prec, const AgentDev *d_ags, int nag, int blocks, hipStream_t st)
+{
+    if (prec == MTR_PREC_BF16) return launch_fp_seg_res_t<__hip_bfloat16>(d_ags, nag, blocks, st);
+    if (prec == MTR_PREC_FP16) return launch_fp_seg_res_t<__half>(d_ags, nag, blocks, st);
+    return launch_fp_seg_res_t<float>(d_ags, nag, blocks, st);
 }
 
 template <class T, int G>

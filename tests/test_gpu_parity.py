@@ -484,6 +484,53 @@ def test_stream_fp_columns_per_wave_give_identical_sums(mtr, orc, name, kw):
     assert orc.rel_l2(outs[0][0], ref) < SOLVE_TOL
 
 
+SEG_GEOMS = [
+    ("slab0", dict(phis=[0.0], Z=21)),
+    ("slab90", dict(phis=[90.0], Z=19)),
+    ("wide", dict(phis=[0.0], X=70, Y=66, Z=23, nu=83, nv=90, views=21)),
+    ("ragged", dict(phis=[0.0], X=37, Y=23, Z=17, nu=29, nv=41)),
+    ("origin", dict(phis=[90.0], Z=18, origin=(5, -3, 2))),
+    ("axisZ", dict(phis=[0.0], kind=1, nu=27)),
+]
+
+
+@pytest.mark.parametrize("name,kw", SEG_GEOMS, ids=[g[0] for g in SEG_GEOMS])
+def test_stream_fp_in_z_segments(mtr, orc, name, kw, monkeypatch):
+    # the streaming FP of a slab in Z segments (4 planes here, 64 at full
+    # size): each segment sums its taps in (line, point) order into partial
+    # ray sums that a second kernel adds in segment order and turns into
+    # the residual.  Against whole rays (FP32 reassociation only) and the
+    # oracle; an agent whose plane axis is Z is never segmented.
+    pb = problem(mtr, **kw)
+    # nonzero data on every pixel, also on rows of planes outside the box
+    # (no voxel: r = p there)
+    pb.ps[0] = (pb.ps[0] + np.random.default_rng(9).uniform(0.0, 0.05, pb.ps[0].shape)).astype(np.float32)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    outs = {}
+    for seg in ("0", "4"):
+        monkeypatch.setenv("MTR_FP_STREAM_SEG", seg)
+        say("stream segments", name, seg)
+        with load_ctx(mtr, pb) as c:
+            assert c.agent_kernels(0)[0] == mtr.FPK_STREAM
+            c.timing(True)
+            x0 = c.bpt(0)
+            x = c.sirt(0, 1, 0.5)
+            outs[seg] = (x0, x, c.residual_norm2(0), c.timing_get(mtr.K_FP_STREAM_RES)[0])
+    assert outs["0"][3] == 0
+    assert (outs["4"][3] > 0) == (kw.get("kind") != 1)
+    assert np.array_equal(outs["0"][0], outs["4"][0])  # BPT: no FP
+    # residual ||p - s/c||^2 of the BPT volume against FP64 numpy on the
+    # oracle's sums
+    s0, c0 = orc.fp(prj, vol, xc, yc, outs["4"][0])
+    r = orc.residual(p, s0, c0).astype(np.float64)
+    ref_r2 = float(np.dot(r.ravel(), r.ravel()))
+    assert outs["4"][2] == pytest.approx(ref_r2, rel=1e-5)
+    assert outs["4"][2] == pytest.approx(outs["0"][2], rel=1e-6)
+    assert orc.rel_l2(outs["4"][1], outs["0"][1]) < OP_TOL
+    ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), 1, 0.5)
+    assert orc.rel_l2(outs["4"][1], ref) < SOLVE_TOL
+
+
 def test_column_fp_row_window_on_a_wide_detector(mtr, orc):
     # a 1000-column detector: whole rows would leave fewer than six blocks
     # per CU, so each wave keeps a window of its rows that the strip walk
