@@ -338,6 +338,7 @@ struct Agent {
     int fp_g = 1;          // streaming forward projection: detector columns per wave (1, 2)
     int fp_seg = 1, fp_seg_len = 0;  // streaming forward projection: Z segments and their planes
     float *d_seg = nullptr;          // per-segment partial ray sums [fp_seg][P] (fp_seg > 1)
+    int *d_seg_ord = nullptr;        // per segment, (view, column group) pairs in crossing order
     int npartial = 0;      // per-block partials written by the last FP launch
     int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
     int64_t st[3] = {0, 0, 0};
@@ -484,7 +485,7 @@ int fp_col_slots(const Agent &ag) { return ag.col_win > 0 ? ag.col_win : ag.prj.
 
 AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
 {
-    AgentDev d;
+    AgentDev d{};
     d.nu = ag.prj.X;
     d.nv = ag.prj.Y;
     d.A = ag.prj.AngN;
@@ -527,6 +528,7 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.seg_part = ag.d_seg;
     d.nseg = ag.fp_seg;
     d.seg_len = ag.fp_seg_len;
+    d.seg_order = ag.d_seg_ord;
     return d;
 }
 
@@ -1117,7 +1119,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
                         (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
-                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_seg})
+                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_seg, (void *)g.d_seg_ord})
             if (q) (void)hipFree(q);
         ctx->agents.pop_back();
         return rc;
@@ -1138,17 +1140,49 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     ag.fp_g = 2;
     if (const char *g = std::getenv("MTR_FP_COLS")) ag.fp_g = std::atoi(g) == 1 ? 1 : 2;
     ag.fast = ag.fp_nc ? sW : 0;
-    // Streaming FP of a slab (plane axis X or Y) in Z segments of 64 planes
-    // when the box has at least two: the rays a block's views share then
-    // fit one XCD's L2.  MTR_FP_STREAM_SEG sets the planes per segment (0:
-    // whole rays).
+    // Streaming FP of a slab (plane axis X or Y) in Z segments of 128
+    // planes when the box has at least two: the voxels that the rays in
+    // flight on an XCD share then mostly stay in its L2 (64 planes halve
+    // that working set but double the partial sums; measured slower at
+    // 1024 x 1024 x 256).  MTR_FP_STREAM_SEG sets the planes per segment
+    // (0: whole rays).
     if (ag.fp_nc && sW != 2) {
-        int len = 64;
+        int len = 128;
         if (const char *g = std::getenv("MTR_FP_STREAM_SEG")) len = std::max(0, std::atoi(g));
         const int nz = ctx->vol.Zend - ctx->vol.Zstart;
         if (len > 0 && nz >= 2 * len) {
             ag.fp_seg = (nz + len - 1) / len;
             ag.fp_seg_len = len;
+        }
+    }
+    // Block order of the streaming FP: in each segment, the (view, column
+    // group) pairs sorted by the coordinate H (the in-plane axis other than
+    // Z) where the group's middle ray crosses the segment's middle plane,
+    // so the blocks in flight on an XCD read one window of H for all views.
+    // MTR_FP_STREAM_ORDER=0 keeps the view-fastest order.
+    std::vector<int> seg_ord;
+    if (ag.fp_nc && sW != 2) {
+        const char *o = std::getenv("MTR_FP_STREAM_ORDER");
+        if (!o || std::atoi(o) != 0) {
+            const int G4 = 4 * ag.fp_g, nti = (prj->X + G4 - 1) / G4, nva = nti * A, H = sW == 0 ? 2 : 1;
+            const double hmid = 0.5 * (sW == 0 ? ctx->vol.Ystart + ctx->vol.Yend - 1 : ctx->vol.Xstart + ctx->vol.Xend - 1);
+            const int len = ag.fp_seg > 1 ? ag.fp_seg_len : ctx->vol.Zend - ctx->vol.Zstart;
+            std::vector<std::pair<double, int>> key(nva);
+            for (int sg = 0; sg < ag.fp_seg; sg++) {
+                const double zc = ctx->vol.Zstart + sg * len + 0.5 * (std::min(len, ctx->vol.Zend - ctx->vol.Zstart - sg * len) - 1);
+                for (int bi = 0; bi < nti; bi++)
+                    for (int v = 0; v < A; v++) {
+                        const double *c = xc + 4 * v;
+                        // u = c0 + cH H + cZ Z - 1 at the group's middle column
+                        const double u = bi * G4 + 0.5 * (G4 - 1);
+                        const double h = std::fabs(c[H]) > 1e-6 ? (u + 1.0 - c[0] - c[3] * zc) / c[H] : hmid;
+                        key[bi * A + v] = {h, v + A * bi};
+                    }
+                std::stable_sort(key.begin(), key.end(),
+                                 [](const std::pair<double, int> &p, const std::pair<double, int> &q) { return p.first < q.first; });
+                for (int q = 0; q < nva; q++) seg_ord.push_back(key[q].second);
+                for (int q = 0; q < nva; q++) key[q] = {0.0, 0};
+            }
         }
     }
     if (!ag.fp_nc && !ag.bpm.sep) {
@@ -1305,6 +1339,11 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                                                   1));
     ALLOC(ag.d_scalar, sizeof(double));
     if (ag.fp_seg > 1) ALLOC(ag.d_seg, sizeof(float) * P * ag.fp_seg);
+    if (!seg_ord.empty()) {
+        ALLOC(ag.d_seg_ord, sizeof(int) * seg_ord.size());
+        if (hipMemcpy(ag.d_seg_ord, seg_ord.data(), sizeof(int) * seg_ord.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return undo(fail(MTR_ERR_HIP, "block order upload failed"));
+    }
     if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess ||
         hipMemcpyAsync(ag.d_plan, plan.data(), sizeof(FpPlan) * A, hipMemcpyHostToDevice, ctx->stream) !=
@@ -1574,7 +1613,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
-                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_seg})
+                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_seg, (void *)ag.d_seg_ord})
             if (q) (void)hipFree(q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,

@@ -100,6 +100,10 @@ struct AgentDev {
     // added and turned into the residual by a second kernel.
     float *seg_part;
     int nseg, seg_len;
+    // Streaming forward projection: per segment, the (view, column group)
+    // pairs ordered by where their rays cross the segment's middle plane,
+    // packed as view + A * group (null: view fastest)
+    const int *seg_order;
 };
 
 }  // namespace mtr

@@ -426,11 +426,12 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
     const AgentDev &ag = ags[blockIdx.y];
     const int nti = (ag.nu + 4 * G - 1) / (4 * G), ntk = (ag.nv + 63) / 64, nseg = ag.nseg;
     // XCD-aware order: each XCD takes a contiguous range of (plane group,
-    // Z segment, column group, view) with the view fastest, so the rays in
-    // flight on one XCD are all views of a few adjacent columns of one
-    // plane group, cut to one segment of seg_len planes: the voxels they
-    // share (about 2 MB for 64 planes) stay in that XCD's L2 across views.
-    const int nbt = nti * ntk * nseg * ag.A, per = (nbt + 7) / 8;
+    // Z segment, column group and view) so that the rays in flight on one
+    // XCD, cut to one segment of seg_len planes, share their voxels in that
+    // XCD's L2.  Within a segment the (view, column group) pairs go in the
+    // host's seg_order (by where the rays cross the segment's middle
+    // plane: all views of one narrow window), else view fastest.
+    const int nva = nti * ag.A, nbt = nva * ntk * nseg, per = (nbt + 7) / 8;
     if (blockIdx.x >= (unsigned)(per * 8)) return;  // beyond this agent's partial sums
     int b = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (b >= nbt) {
@@ -438,12 +439,13 @@ __global__ __launch_bounds__(256) void k_fp_stream(const AgentDev *__restrict__ 
         if (threadIdx.x == 0 && nseg == 1) ag.partial[blockIdx.x] = 0.0;
         return;
     }
-    const int a = b % ag.A;
-    b /= ag.A;
-    const int bi = b % nti;
-    b /= nti;
+    int va = b % nva;
+    b /= nva;
     const int sg = b % nseg;
     const int bk = b / nseg;
+    if (ag.seg_order) va = ag.seg_order[sg * nva + va];
+    const int a = va % ag.A;
+    const int bi = va / ag.A;
     const int lane = threadIdx.x & 63, wid = wave_id();
     const int i0 = (bi * 4 + wid) * G;
     const int gn = min(G, ag.nu - i0);  // this wave's columns on the detector (may be <= 0)

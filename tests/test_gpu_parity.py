@@ -496,7 +496,7 @@ SEG_GEOMS = [
 
 @pytest.mark.parametrize("name,kw", SEG_GEOMS, ids=[g[0] for g in SEG_GEOMS])
 def test_stream_fp_in_z_segments(mtr, orc, name, kw, monkeypatch):
-    # the streaming FP of a slab in Z segments (4 planes here, 64 at full
+    # the streaming FP of a slab in Z segments (4 planes here, 128 at full
     # size): each segment sums its taps in (line, point) order into partial
     # ray sums that a second kernel adds in segment order and turns into
     # the residual.  Against whole rays (FP32 reassociation only) and the
@@ -507,15 +507,21 @@ def test_stream_fp_in_z_segments(mtr, orc, name, kw, monkeypatch):
     pb.ps[0] = (pb.ps[0] + np.random.default_rng(9).uniform(0.0, 0.05, pb.ps[0].shape)).astype(np.float32)
     prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
     outs = {}
-    for seg in ("0", "4"):
+    for seg, order in (("0", "1"), ("4", "1"), ("4", "0")):
+        # block order: by ray crossing (1) or view fastest (0)
         monkeypatch.setenv("MTR_FP_STREAM_SEG", seg)
-        say("stream segments", name, seg)
+        monkeypatch.setenv("MTR_FP_STREAM_ORDER", order)
+        say("stream segments", name, seg, order)
         with load_ctx(mtr, pb) as c:
             assert c.agent_kernels(0)[0] == mtr.FPK_STREAM
             c.timing(True)
             x0 = c.bpt(0)
             x = c.sirt(0, 1, 0.5)
-            outs[seg] = (x0, x, c.residual_norm2(0), c.timing_get(mtr.K_FP_STREAM_RES)[0])
+            outs[seg if order == "1" else "4v"] = (x0, x, c.residual_norm2(0),
+                                                    c.timing_get(mtr.K_FP_STREAM_RES)[0])
+    # the block order changes which block sums a ray, not its sum
+    assert np.array_equal(outs["4"][1], outs["4v"][1])
+    assert outs["4"][2] == pytest.approx(outs["4v"][2], rel=1e-12)
     assert outs["0"][3] == 0
     assert (outs["4"][3] > 0) == (kw.get("kind") != 1)
     assert np.array_equal(outs["0"][0], outs["4"][0])  # BPT: no FP
