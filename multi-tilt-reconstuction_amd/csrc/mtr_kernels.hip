@@ -1711,9 +1711,18 @@ __device__ __forceinline__ void pfx_eval(ColRows &rw, const double2 *m, int nz, 
                                          double inv, float wa, float wb, int lane)
 {
     constexpr int PP = 64 * R - 2;
-    const double sb = -ub * inv, lanefR = (double)(R * lane);
-    for (int p = R == 2 ? (ilo & ~1) : ilo; p <= ihi; p += PP) {
-        const double s0 = fma(lanefR + (double)(p - 1), inv, sb);
+    const double sb = -ub * inv;
+    const int p0 = R == 2 ? (ilo & ~1) : ilo;
+    // boundary of the lane's first slot, m = p - 1 + R lane, kept as an
+    // exact FP64 integer across passes
+    double mpos = (double)(p0 - 1 + R * lane);
+    // slots outside [ilo, ihi] add zero, and so do lane 0's first and
+    // lane 63's last slot
+    const unsigned span = (unsigned)(ihi - ilo);
+    const int rel = R * lane - 1 - ilo;
+    const bool edge_lo = lane == 0, edge_hi = lane == 63;
+    for (int p = p0; p <= ihi; p += PP, mpos += (double)PP) {
+        const double s0 = fma(mpos, inv, sb);
         double X2[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -1729,17 +1738,18 @@ __device__ __forceinline__ void pfx_eval(ColRows &rw, const double2 *m, int nz, 
         }
         const double left = dpp_d<0x138>(X2[R - 1]);  // lane - 1's last (wave_shr 1)
         const double right = dpp_d<0x130>(X2[0]);     // lane + 1's first (wave_shl 1)
-        const int sl0 = p + R * lane;
         float t[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const double xm = r == 0 ? left : X2[r - 1];
             const double xp = r == R - 1 ? right : X2[r + 1];
-            t[r] = (float)(fma(-2.0, X2[r], xm) + xp);
-            if ((r == 0 && lane == 0) || (r == R - 1 && lane == 63) ||
-                (unsigned)(sl0 + r - 1 - ilo) > (unsigned)(ihi - ilo))
-                t[r] = 0.0f;
+            const float tv = (float)(fma(-2.0, X2[r], xm) + xp);
+            bool zero = (unsigned)(p + rel + r) > span;
+            if (r == 0) zero = zero || edge_lo;
+            if (r == R - 1) zero = zero || edge_hi;
+            t[r] = zero ? 0.0f : tv;
         }
+        const int sl0 = p + R * lane;
         const int g = WIN ? (sl0 & rw.msk) : sl0;
         if constexpr (R == 1) {
             const float2 rv = rw.rw[g];
@@ -1771,7 +1781,7 @@ __global__ __launch_bounds__(64 * kPfxWaves) void k_fp_pfx(const AgentDev *__res
     if (UNIT && !view) return;  // no block-wide barrier without staged moments
     const int nu = ag.nu, nv = ag.nv, L = nu + 2, nz = ag.Z, zs = ag.Zs;
     const int nslot = pfx_slots(WIN ? ag.col_win : 0, L);
-    const int ncol = UNIT ? kPfxMaxCols : pfx_cols(nslot, nz);
+    const int ncol = __builtin_amdgcn_readfirstlane(UNIT ? kPfxMaxCols : pfx_cols(nslot, nz));
     float2 *rows = reinterpret_cast<float2 *>(pfx_raw);
     double2 *mom = reinterpret_cast<double2 *>(rows + kPfxWaves * nslot);
     const int k0 = s - 1;
@@ -1798,7 +1808,7 @@ __global__ __launch_bounds__(64 * kPfxWaves) void k_fp_pfx(const AgentDev *__res
     // (|c3| (Z - 1) + 3 at most), two otherwise (64 R slots in the window)
     const int npix = (int)(fabs(c3) * (double)(nz - 1)) + 3;
     const int R = __builtin_amdgcn_readfirstlane(npix <= 62 || (WIN && nslot < 128) ? 1 : 2);
-    const int j0 = ag.strip[s], j1 = ag.strip[s + 1];
+    const int j0 = __builtin_amdgcn_readfirstlane(ag.strip[s]), j1 = __builtin_amdgcn_readfirstlane(ag.strip[s + 1]);
     const auto x = gview(reinterpret_cast<const T *>(ag.x));
     const int64_t xoff = -(int64_t)ag.Xs * ag.sx - (int64_t)ag.Ys * ag.sy;
     auto column = [&](int2 xy) { return x + (xoff + (int64_t)xy.x * ag.sx + (int64_t)xy.y * ag.sy); };
@@ -1814,7 +1824,8 @@ __global__ __launch_bounds__(64 * kPfxWaves) void k_fp_pfx(const AgentDev *__res
     }
     if (!UNIT && wid < min(ncol, j1 - j0)) pfx_load<T>(column(ucol_xy(ag.col_xy, j0 + wid)), 0, nz, lane, xpre);
     for (int jr = j0; jr < j1; jr += ncol) {
-        const int nc = min(ncol, j1 - jr), jn = jr + ncol, ncn = max(min(ncol, j1 - jn), 0);
+        const int nc = __builtin_amdgcn_readfirstlane(min(ncol, j1 - jr)), jn = jr + ncol,
+                  ncn = __builtin_amdgcn_readfirstlane(max(min(ncol, j1 - jn), 0));
         if constexpr (!UNIT) {
             if (wid < nc) pfx_column<T>(column(ucol_xy(ag.col_xy, jr + wid)), nz, mom + wid * zp, lane, xpre);
             __syncthreads();
