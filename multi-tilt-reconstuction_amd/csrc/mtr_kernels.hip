@@ -2304,6 +2304,7 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         const int ib = (int)fb - cfi0;
         const float4 *stg = lds4 + (a & 1) * kCsMaxW;
         const int lmax = min(cfi1 - cfi0, kCsMaxW - 1);
+        const float ibf = (float)ib, lmaxf = (float)lmax;
         auto taps = [&](auto off_c) {
             constexpr int OFF = decltype(off_c)::value;
             if constexpr (!NORM) {
@@ -2312,7 +2313,8 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
                     const float g = fmaf((float)(OFF + t), step, f0);
                     const float fg = floorf(g);
                     const float du = g - fg;
-                    const int li = min(max(ib + (int)fg, 0), lmax);
+                    // clamp in FP32 on exact integers: one med3
+                    const int li = (int)__builtin_amdgcn_fmed3f(fg + ibf, 0.0f, lmaxf);
                     const float4 q = stg[li];
                     S[t] = fmaf(wk0, fmaf(du, q.y, q.x), S[t]);
                     S[t] = fmaf(wk1, fmaf(du, q.w, q.z), S[t]);
