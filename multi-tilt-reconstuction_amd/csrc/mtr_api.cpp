@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -400,6 +401,7 @@ struct mtr_ctx {
     float *d_s = nullptr, *d_cc = nullptr;
     int64_t sc_cap = 0;
     bool autotune = true;
+    bool tune_timed = true;  // false: tiles from the tuning cache only, never timed here
     bool joint_tune = true;  // the joint run-length stage after the initial model
     std::string joint_key;   // agents it last ran for
     // graph of one fast MACE step, replayed for every step of a solve
@@ -478,6 +480,15 @@ int collect_timing(mtr_ctx *ctx)
     }
     ctx->pending.clear();
     return MTR_OK;
+}
+
+// Host-to-device copy of a host buffer that is freed after the call,
+// ordered on the context's stream (a null-stream copy would also wait for
+// every other blocking stream of the device).
+hipError_t upload_now(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
 }
 
 // LDS row slots per wave of the column-plan FP.
@@ -867,6 +878,10 @@ int tune_agent(mtr_ctx *ctx, int a)
             ag.tuned = true;
             return apply_tiles(ctx, a, it->second.first, it->second.second);
         }
+    }
+    if (!ctx->tune_timed) {  // the heuristic tiles bp_mode chose
+        ag.tuned = true;
+        return MTR_OK;
     }
     const bool timing = ctx->timing;
     ctx->timing = false;
@@ -1323,14 +1338,13 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
             if (ag.col_cs) {
                 ag.ngrp = (int)grp.size();
                 ALLOC(ag.d_col_grp, sizeof(ColGroup) * grp.size());
-                if (hipMemcpy(ag.d_col_grp, grp.data(), sizeof(ColGroup) * grp.size(), hipMemcpyHostToDevice) !=
-                    hipSuccess)
+                if (upload_now(ag.d_col_grp, grp.data(), sizeof(ColGroup) * grp.size(), ctx->stream) != hipSuccess)
                     return undo(fail(MTR_ERR_HIP, "column group upload failed"));
             }
         }
-        if (hipMemcpy(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(ag.d_col_w, wts.data(), sizeof(float2) * wts.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(ag.d_strip, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (upload_now(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), ctx->stream) != hipSuccess ||
+            upload_now(ag.d_col_w, wts.data(), sizeof(float2) * wts.size(), ctx->stream) != hipSuccess ||
+            upload_now(ag.d_strip, cnt.data(), sizeof(int) * cnt.size(), ctx->stream) != hipSuccess)
             return undo(fail(MTR_ERR_HIP, "column table upload failed"));
     }
     ag.nfp = fp_blocks(make_dev(ctx, ag), ag.lanes_k);
@@ -1341,7 +1355,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     if (ag.fp_seg > 1) ALLOC(ag.d_seg, sizeof(float) * P * ag.fp_seg);
     if (!seg_ord.empty()) {
         ALLOC(ag.d_seg_ord, sizeof(int) * seg_ord.size());
-        if (hipMemcpy(ag.d_seg_ord, seg_ord.data(), sizeof(int) * seg_ord.size(), hipMemcpyHostToDevice) != hipSuccess)
+        if (upload_now(ag.d_seg_ord, seg_ord.data(), sizeof(int) * seg_ord.size(), ctx->stream) != hipSuccess)
             return undo(fail(MTR_ERR_HIP, "block order upload failed"));
     }
     if (hipMemcpyAsync(ag.d_coef, coef.data(), sizeof(ViewCoef) * A, hipMemcpyHostToDevice, ctx->stream) !=
@@ -2596,6 +2610,9 @@ int mtr_sync(mtr_ctx *ctx)
 namespace {
 
 enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3 };
+// host-side preparation of a pending request (its context, tables and
+// uploads) by the server's worker thread
+enum { PREP_NONE = 0, PREP_QUEUED = 1, PREP_RUNNING = 2, PREP_DONE = 3, PREP_FAILED = 4 };
 
 struct ServerReq {
     int64_t id = 0;
@@ -2614,6 +2631,11 @@ struct ServerReq {
     std::vector<float> result;
     std::string error;
     int code = MTR_ERR_HIP;  // status of a failed request
+    // PREP_*: written under mtr_server::mu (the tick thread sets QUEUED and
+    // NONE, the worker RUNNING, DONE and FAILED)
+    int prep = PREP_NONE;
+    int prep_rc = MTR_OK;
+    std::string prep_err;
 };
 
 }  // namespace
@@ -2635,6 +2657,17 @@ struct mtr_server {
     AgentDev *d_tab = nullptr;
     int tab_cap = 0;
     std::vector<AgentDev> h_tab;
+    // Background preparation: a worker thread builds the contexts of the
+    // requests the admission policy takes next (allocations, geometry
+    // tables, projection uploads on prep_stream) while the GPU runs ticks;
+    // admission then only launches their normalisers and initial models.
+    bool prefetch = true;
+    hipStream_t prep_stream = nullptr;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServerReq *> prep_q;
+    bool quit = false;
     std::chrono::steady_clock::time_point t0;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -2665,17 +2698,20 @@ void release(mtr_server *srv, ServerReq *r)
     srv->used -= std::min(srv->used, r->bytes);
 }
 
-// Creates the request's context on the server's stream, uploads its tilt
-// series and forms the BPT-sum initial model (MultiAxis.cpp:163-200).
-int admit(mtr_server *srv, ServerReq *r)
+// Host side of admission: the request's context on stream st, its
+// agents' geometry tables, allocations and projection uploads; nothing is
+// launched on the engine's stream unless st is that stream.
+int prepare(mtr_server *srv, ServerReq *r, hipStream_t st)
 {
     mtr_ctx *c = new mtr_ctx();
     c->device = srv->device;
-    c->stream = srv->eng->stream;
+    c->stream = st;
     c->own_stream = false;
     c->precision = srv->precision;
     c->joint_tune = false;  // its agents share launches with other requests' agents
+    c->tune_timed = false;  // no timed tuning inside a tick: cached tiles or the heuristic ones
     r->ctx = c;
+    MTR_TRY(set_device(c));
     MTR_TRY(alloc_tables(c));
     MTR_TRY(mtr_ctx_set_volume(c, &r->vol));
     for (size_t a = 0; a < r->prj.size(); a++) {
@@ -2684,9 +2720,96 @@ int admit(mtr_server *srv, ServerReq *r)
                                  true));
         std::vector<float>().swap(r->p[a]);  // resident now
     }
+    HIP_TRY(hipStreamSynchronize(st));
+    return MTR_OK;
+}
+
+// GPU side of admission, on the engine's stream: normalisers and the
+// BPT-sum initial model (MultiAxis.cpp:163-200).
+int activate(mtr_server *srv, ServerReq *r)
+{
+    mtr_ctx *c = r->ctx;
+    c->stream = srv->eng->stream;
     for (int a = 0; a < (int)c->agents.size(); a++) MTR_TRY(ensure_normaliser(c, a));
     MTR_TRY(mtr_mace_init(c));
     return MTR_OK;
+}
+
+void prep_worker(mtr_server *srv)
+{
+    (void)hipSetDevice(srv->device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        srv->cv_work.wait(lk, [&] { return srv->quit || !srv->prep_q.empty(); });
+        if (srv->quit) return;
+        ServerReq *r = srv->prep_q.front();
+        srv->prep_q.pop_front();
+        r->prep = PREP_RUNNING;
+        lk.unlock();
+        const int rc = prepare(srv, r, srv->prep_stream);
+        std::string err = rc != MTR_OK ? g_err : std::string();
+        lk.lock();
+        r->prep_rc = rc;
+        r->prep_err = std::move(err);
+        r->prep = rc == MTR_OK ? PREP_DONE : PREP_FAILED;
+        srv->cv_done.notify_all();
+    }
+}
+
+// The pending requests in the order the admission loop would take them
+// now: the oldest, or (smallest first) the least work unless the oldest
+// has waited longer than max_wait.
+std::vector<ServerReq *> policy_order(const mtr_server *srv)
+{
+    std::vector<ServerReq *> left(srv->pending.begin(), srv->pending.end()), out;
+    const double now = srv->now();
+    while (!left.empty()) {
+        size_t pick = 0;
+        if (srv->admit_policy == MTR_ADMIT_SMALLEST && now - left[0]->t_submit <= srv->max_wait)
+            for (size_t q = 1; q < left.size(); q++)
+                if (left[q]->work < left[pick]->work) pick = q;
+        out.push_back(left[pick]);
+        left.erase(left.begin() + (long)pick);
+    }
+    return out;
+}
+
+// Queues for preparation the requests the policy takes next, for the slots
+// the active ones leave free and one more, while the HBM budget allows
+// (their bytes count as in use from here on).
+void plan_prep(mtr_server *srv)
+{
+    if (!srv->prefetch) return;
+    const std::vector<ServerReq *> order = policy_order(srv);
+    const int want = std::max(srv->max_active - (int)srv->active.size(), 0) + 1;
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (int q = 0; q < (int)order.size() && q < want; q++) {
+        ServerReq *r = order[q];
+        if (r->prep != PREP_NONE) continue;
+        if (srv->used + r->bytes > srv->budget) break;
+        srv->used += r->bytes;
+        r->prep = PREP_QUEUED;
+        srv->prep_q.push_back(r);
+    }
+    if (!srv->prep_q.empty()) srv->cv_work.notify_one();
+}
+
+// The admitted request's prepared context: waits for the worker if it is
+// preparing it, else prepares it here on the engine's stream.
+int take_prepared(mtr_server *srv, ServerReq *r)
+{
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        if (r->prep == PREP_QUEUED) {
+            srv->prep_q.erase(std::find(srv->prep_q.begin(), srv->prep_q.end(), r));
+            r->prep = PREP_NONE;
+        } else if (r->prep == PREP_RUNNING) {
+            srv->cv_done.wait(lk, [&] { return r->prep == PREP_DONE || r->prep == PREP_FAILED; });
+        }
+        if (r->prep == PREP_FAILED) return fail(r->prep_rc, "%s", r->prep_err.c_str());
+        if (r->prep == PREP_DONE) return MTR_OK;
+    }
+    return prepare(srv, r, srv->eng->stream);
 }
 
 }  // namespace
@@ -2710,6 +2833,15 @@ int mtr_server_create(int device, int precision, size_t hbm_budget, int max_acti
     srv->max_active = max_active;
     srv->eng = eng;
     srv->t0 = std::chrono::steady_clock::now();
+    if (const char *pf = std::getenv("MTR_SERVER_PREFETCH")) srv->prefetch = std::atoi(pf) != 0;
+    if (srv->prefetch) {
+        if (hipStreamCreateWithFlags(&srv->prep_stream, hipStreamNonBlocking) != hipSuccess) {
+            mtr_ctx_destroy(eng);
+            delete srv;
+            return fail(MTR_ERR_HIP, "server preparation stream");
+        }
+        srv->worker = std::thread(prep_worker, srv);
+    }
     *out = srv;
     return MTR_OK;
 }
@@ -2718,8 +2850,17 @@ int mtr_server_destroy(mtr_server *srv)
 {
     if (!srv) return MTR_OK;
     (void)hipSetDevice(srv->device);
+    if (srv->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->quit = true;
+        }
+        srv->cv_work.notify_all();
+        srv->worker.join();
+    }
     (void)hipStreamSynchronize(srv->eng->stream);
     for (auto &kv : srv->reqs) release(srv, kv.second.get());
+    if (srv->prep_stream) (void)hipStreamDestroy(srv->prep_stream);
     if (srv->d_tab) (void)hipFree(srv->d_tab);
     mtr_ctx_destroy(srv->eng);
     delete srv;
@@ -2780,6 +2921,7 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
     if (request_id) *request_id = r->id;
     srv->pending.push_back(r.get());
     srv->reqs[r->id] = std::move(r);
+    plan_prep(srv);
     return MTR_OK;
 }
 
@@ -2816,11 +2958,18 @@ int mtr_server_step(mtr_server *srv, int *completed)
             for (auto it = srv->pending.begin(); it != srv->pending.end(); ++it)
                 if ((*it)->work < (*pick)->work) pick = it;
         ServerReq *r = *pick;
-        if (srv->used + r->bytes > srv->budget) break;
+        // a request queued for preparation already counts in `used`
+        size_t need = 0;
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            need = r->prep == PREP_NONE ? r->bytes : 0;
+        }
+        if (srv->used + need > srv->budget) break;
         srv->pending.erase(pick);
-        srv->used += r->bytes;
+        srv->used += need;
         r->t_admit = srv->now();
-        const int rc = admit(srv, r);
+        int rc = take_prepared(srv, r);
+        if (rc == MTR_OK) rc = activate(srv, r);
         if (rc != MTR_OK) {
             // finished, with its error for mtr_server_result
             r->error = g_err;
@@ -2933,6 +3082,8 @@ int mtr_server_step(mtr_server *srv, int *completed)
     }
     srv->active.swap(still);
     MTR_TRY(collect_timing(srv->eng));
+    // the next admissions' host work overlaps the next tick
+    plan_prep(srv);
     if (completed) *completed = done;
     return MTR_OK;
 }
