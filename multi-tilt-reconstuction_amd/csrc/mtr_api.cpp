@@ -482,6 +482,41 @@ int collect_timing(mtr_ctx *ctx)
     return MTR_OK;
 }
 
+// Planes per Z segment of the streaming FP (0: whole rays).
+int fp_seg_planes()
+{
+    int len = 128;
+    if (const char *g = std::getenv("MTR_FP_STREAM_SEG")) len = std::max(0, std::atoi(g));
+    return len;
+}
+
+// Device bytes of the streaming FP's Z segments for an agent of P pixels
+// and A views over nz planes, should it take that kernel: the partial ray
+// sums and the block order.
+size_t fp_seg_bytes(size_t P, const mtr_projection_size &prj, int nz)
+{
+    const int len = fp_seg_planes();
+    if (len <= 0 || nz < 2 * len) return 0;
+    const size_t nseg = (size_t)((nz + len - 1) / len);
+    return 4 * P * nseg + 4 * nseg * (size_t)prj.AngN * (size_t)((prj.X + 7) / 8);
+}
+
+// Upper bound of one agent's device bytes over the kernel choices: geometry
+// tables, p (plus a converted copy), c, r, the partial rows of the column
+// plan, x, w and a canonical copy of x, the column plan's back-projection
+// normaliser and tables, FP64 partial sums, the streaming FP's segments.
+size_t agent_bytes(int precision, const mtr_volume_size &vol, const mtr_projection_size &prj)
+{
+    const size_t es = elem_size(precision);
+    const size_t P = (size_t)prj.X * prj.Y * prj.AngN;
+    const size_t N = (size_t)vol.X * vol.Y * vol.Z;
+    const size_t cols = (size_t)vol.X * vol.Y;
+    const size_t nfp = (size_t)((prj.X + 63) / 64 + 16) * ((prj.Y + 3) / 4 + 16) * prj.AngN;
+    return (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj.AngN + 2 * 4 * P + 4 * P + es * P +
+           2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj.Y + 2) + 8 * nfp +
+           sizeof(ColGroup) * (cols / 64 + (size_t)prj.Y + 2) + fp_seg_bytes(P, prj, vol.Z);
+}
+
 // Host-to-device copy of a host buffer that is freed after the call,
 // ordered on the context's stream (a null-stream copy would also wait for
 // every other blocking stream of the device).
@@ -1162,9 +1197,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     // 1024 x 1024 x 256).  MTR_FP_STREAM_SEG sets the planes per segment
     // (0: whole rays).
     if (ag.fp_nc && sW != 2) {
-        int len = 128;
-        if (const char *g = std::getenv("MTR_FP_STREAM_SEG")) len = std::max(0, std::atoi(g));
-        const int nz = ctx->vol.Zend - ctx->vol.Zstart;
+        const int len = fp_seg_planes(), nz = ctx->vol.Zend - ctx->vol.Zstart;
         if (len > 0 && nz >= 2 * len) {
             ag.fp_seg = (nz + len - 1) / len;
             ag.fp_seg_len = len;
@@ -1723,17 +1756,7 @@ int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size
     if (!ctx || !bytes) return fail(MTR_ERR_ARG, "null argument");
     if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
     if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
-    const size_t es = elem_size(ctx->precision);
-    const size_t P = (size_t)prj->X * prj->Y * prj->AngN;
-    const size_t N = (size_t)ctx->N;
-    const size_t cols = (size_t)ctx->vol.X * ctx->vol.Y;
-    const size_t nfp = (size_t)((prj->X + 63) / 64 + 16) * ((prj->Y + 3) / 4 + 16) * prj->AngN;
-    // geometry tables, p (plus a converted copy), c, r, the partial rows of
-    // the column plan, x, w and a canonical copy of x, the column plan's
-    // back-projection normaliser and tables, FP64 partial sums
-    *bytes = (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj->AngN + 2 * 4 * P + 4 * P + es * P +
-             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj->Y + 2) + 8 * nfp +
-             sizeof(ColGroup) * (cols / 64 + (size_t)prj->Y + 2);
+    *bytes = agent_bytes(ctx->precision, ctx->vol, *prj);
     return MTR_OK;
 }
 
@@ -2678,14 +2701,8 @@ size_t request_bytes(int precision, const mtr_volume_size &vol, int K, const mtr
 {
     const size_t es = elem_size(precision);
     const size_t N = (size_t)vol.X * vol.Y * vol.Z;
-    const size_t cols = (size_t)vol.X * vol.Y;
     size_t b = es * N + 3 * sizeof(AgentDev) * kMaxAgents + 8 * 8193;  // w-bar, tables, consensus partials
-    for (int a = 0; a < K; a++) {
-        const size_t P = (size_t)prj[a].X * prj[a].Y * prj[a].AngN;
-        const size_t nfp = (size_t)((prj[a].X + 63) / 64 + 16) * ((prj[a].Y + 3) / 4 + 16) * prj[a].AngN;
-        b += (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj[a].AngN + 2 * 4 * P + 4 * P + es * P +
-             2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj[a].Y + 2) + 8 * nfp;
-    }
+    for (int a = 0; a < K; a++) b += agent_bytes(precision, vol, prj[a]);
     return b;
 }
 
