@@ -108,6 +108,37 @@ def oracle_slab_times(mtr, orc, pb, agents, depths, keep=None):
     return times, outs
 
 
+def full_single_axis(mtr, threads: int):
+    """The single-axis workload (256x256x64, 61 views, BPT + 10 SIRT
+    iterations: the reference's CPU-runnable case) solved whole by the CPU
+    oracle on `threads` host threads and by the GPU on the same seeded
+    inputs, with the relative L2 difference of the two results."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the CPU restatement (timed baseline only)
+
+    pb = mtr.synth.config("c1", threads=16)
+    prj, vol, xc, yc, p = pb.prjs[0], pb.vol, pb.xcs[0], pb.ycs[0], pb.ps[0]
+    used = orc.set_threads(threads)
+    t0 = time.perf_counter()
+    ref = orc.sirt(prj, vol, xc, yc, p, orc.bpt(prj, vol, xc, yc, p), pb.S, pb.lam)
+    t_cpu = time.perf_counter() - t0
+    with mtr.Context(0) as c:
+        c.set_volume(vol)
+        c.add_agent(prj, xc, yc, p)
+        c.bpt(0, out=False)  # first launches and tile tuning
+        c.sirt(0, pb.S, pb.lam, out=False)
+        c.sync()
+        t0 = time.perf_counter()
+        c.bpt(0, out=False)
+        c.sirt(0, pb.S, pb.lam, out=False)
+        c.sync()
+        t_gpu = time.perf_counter() - t0
+        got = c.get_volume(mtr.VOL_WORK)
+    return {"workload": "c1: single axis 256x256x64, 61 views, BPT + 10 SIRT iterations, whole",
+            "cpu_seconds": t_cpu, "threads": used, "gpu_seconds": t_gpu, "gpu_over_cpu": t_cpu / t_gpu,
+            "rel_l2_gpu_vs_oracle": float(orc.rel_l2(got, ref))}
+
+
 def extrapolate(times, depths, Z):
     """t(nz) = alpha + beta * nz through the two depths, at the full depth Z
     (the per-pixel work of a SIRT iteration does not grow with depth)."""
@@ -586,6 +617,8 @@ def main() -> int:
         cpu, (nz, outs) = cpu_baseline(mtr, pb, threads)
         log(f"cpu baseline: {cpu['seconds_per_reconstruction']:.0f} s per reconstruction with {cpu['threads']} "
             f"threads, {cpu['one_core']['seconds_per_reconstruction']:.0f} s with one")
+        cpu["full_single_axis"] = full_single_axis(mtr, threads)
+        log(f"single-axis workload whole: {cpu['full_single_axis']}")
         if prec == mtr.PREC_FP32:
             parity["slab_sample"] = slab_parity(mtr, pb, nz, outs)
             log(f"slab parity: {parity['slab_sample']}")
