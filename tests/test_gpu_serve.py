@@ -156,3 +156,29 @@ def test_zero_step_request_returns_the_initial_model(mtr):
             c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
         ref = c.mace_solve(0, pb.S, pb.lam, pb.rho)
     assert np.array_equal(got, ref)
+
+
+def test_background_preparation_takes_memory_at_submission(mtr, monkeypatch):
+    """With background preparation (the default) a submitted request is
+    queued for the worker at once, so its footprint counts as in use before
+    any tick; with MTR_SERVER_PREFETCH=0 nothing is taken until the tick
+    that admits it.  Both give the same result, and a mix of requests
+    prepared while others run equals the same mix prepared inside ticks."""
+    pbs = [req(mtr, 40, 40, 12, [0.0, 90.0], 80, M=2), req(mtr, 32, 28, 10, [45.0], 81, M=3),
+           req(mtr, 36, 36, 12, [0.0, 60.0, 120.0], 82, M=2)]
+    res = {}
+    for pf in ("1", "0"):
+        monkeypatch.setenv("MTR_SERVER_PREFETCH", pf)
+        say("server preparation", pf)
+        with mtr.Server(0, max_active=2) as srv:
+            need = srv.footprint(pbs[0].vol, pbs[0].prjs)
+            ids = [srv.submit_problem(pbs[0])]
+            used = srv.status()["hbm_in_use"]
+            assert used == (need if pf == "1" else 0)
+            srv.step()
+            ids += [srv.submit_problem(p) for p in pbs[1:]]
+            drain(srv, 3)
+            assert srv.status()["hbm_in_use"] == 0
+            res[pf] = [srv.result(r)[0] for r in ids]
+    for a, b in zip(res["1"], res["0"]):
+        assert np.array_equal(a, b)
