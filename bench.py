@@ -563,7 +563,9 @@ def main() -> int:
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": d["valu_frac"],
-        "traffic": tr,
+        # HBM bytes per launch (FETCH + WRITE, each scaled by the copy calibration), and the two counters
+        "traffic": (tr["fetch_bytes"] + tr["write_bytes"]) if tr else None,
+        "traffic_detail": tr,
         "share_of_step": d["total_ms"] / step_kernel_ms,
         "agents_per_launch": d["agents_per_launch"],
         "vvu_per_launch": d["vvu_per_launch"],
