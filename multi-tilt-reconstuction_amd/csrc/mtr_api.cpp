@@ -2857,7 +2857,13 @@ int mtr_server_create(int device, int precision, size_t hbm_budget, int max_acti
             delete srv;
             return fail(MTR_ERR_HIP, "server preparation stream");
         }
-        srv->worker = std::thread(prep_worker, srv);
+        try {
+            srv->worker = std::thread(prep_worker, srv);
+        } catch (...) {  // no thread to spare: prepare inside the ticks
+            (void)hipStreamDestroy(srv->prep_stream);
+            srv->prep_stream = nullptr;
+            srv->prefetch = false;
+        }
     }
     *out = srv;
     return MTR_OK;
