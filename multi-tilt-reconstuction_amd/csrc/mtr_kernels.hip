@@ -409,6 +409,14 @@ __global__ __launch_bounds__(256) void k_fp(const AgentDev *__restrict__ ags, fl
 // sums its own taps in (line, point) order, the order of the per-lane path,
 // with the same weight expressions, so the sums are those of the per-lane
 // path for every G (the extra FMAs add exact zeros).
+// Deep boxes (ag.nseg > 1, plane axis X or Y) are cut into Z segments of
+// seg_len planes: a block sums its rays' taps inside one segment (the same
+// order, restricted to the segment's lines or lattice points) and writes
+// them as FP32 partial sums, which k_fp_seg_res adds in segment order and
+// turns into the residual; the fused epilogue below serves whole rays.
+// The rays in flight on an XCD then share a few MB of voxels instead of
+// tens, which its L2 holds.
+
 // Binary digits of n (n >= 0).
 __host__ __device__ constexpr int bits_for(int n) { return n < 2 ? 1 : 1 + bits_for(n / 2); }
 
