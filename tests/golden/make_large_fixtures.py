@@ -16,7 +16,7 @@ output voxels is written with per-Z-plane sums and the squared norm:
     p_sha256   hash of every agent's projections (the test checks that it
                regenerated the same inputs before comparing)
 
-Run from the repository root (about 50 minutes on 8 cores, 20 GB of memory):
+Run from the repository root (about 50 minutes per full-size case on 8 cores, 20 GB of memory):
     python tests/golden/make_large_fixtures.py [case ...]
 """
 from __future__ import annotations
@@ -41,6 +41,9 @@ CASES = {
     "c3_m1s1": ("c3", {}, 1, 1),
     # the dual-axis workload's full schedule (10 MACE steps x 5 SIRT) on a 256x256x64 box, 61 views
     "c2_sched_256": ("c2", dict(X=256, Y=256, Z=64, n_ell=100), 10, 5),
+    # the three-axis workload (axes 0/60/120, 1024x1024x256, 61 views) at one MACE step of one SIRT
+    # iteration: the split-precision modes against the FP64 oracle at full size
+    "c4_m1s1": ("c4", {}, 1, 1),
 }
 N_SAMPLE = 32768
 SAMPLE_SEED = 20261204

@@ -8,7 +8,8 @@ FunctionOnGPU.cu:212-244), in both oracle modes:
   four-axis (0/45/90/135 deg), 61 views   128x128x32, M=2, S=2, oracle run here
   four-axis, full size 1024x1024x256       M=1, S=1, against tests/golden/large_c3_m1s1.npz
   dual-axis (0/90 deg) full schedule       256x256x64, M=10, S=5, against large_c2_sched_256.npz
-  three-axis (0/60/120 deg) in FP32, BF16 and FP16 storage against the FP64 oracle
+  three-axis (0/60/120 deg) in FP32, BF16 and FP16 storage against the FP64 oracle,
+      at 128x128x32 (M=2, S=2) and at full size (M=1, S=1, large_c4_m1s1.npz)
 
 Bars (relative L2 = ||gpu - oracle|| / ||oracle||): FP32 storage <= 1e-5
 against both oracle modes; BF16 <= 2e-2 and FP16 <= 3e-3 against FP64 (the
@@ -136,3 +137,36 @@ def test_three_axis_split_precision_against_fp64_oracle(mtr, orc, three_axis, na
         f"(bound {SPLIT_BOUND[name]:.0e})")
     assert np.isfinite(out).all()
     assert err < SPLIT_BOUND[name]
+
+
+@pytest.fixture(scope="module")
+def three_axis_full(mtr):
+    fx = fixtures()
+    path = os.path.join(GOLD, "large_c4_m1s1.npz")
+    if not os.path.exists(path):
+        pytest.skip("large_c4_m1s1.npz not generated (tests/golden/make_large_fixtures.py c4_m1s1)")
+    g = np.load(path)
+    pb, M, S = fx.problem(mtr, "c4_m1s1")
+    assert str(g["p_sha256"]) == fx.projections_hash(pb), "generator output differs from the fixture's inputs"
+    return pb, M, S, g
+
+
+@pytest.mark.parametrize("name", ["fp32", "bf16", "fp16"])
+def test_three_axis_full_size_split_precision_against_fp64_oracle(mtr, orc, three_axis_full, name):
+    # 1024x1024x256, axes 0/60/120, 61 views, one MACE step of one SIRT
+    # iteration, in each storage precision against the FP64 oracle's sample
+    # of w-bar, plane sums and norm (bars as at the reduced size)
+    pb, M, S, g = three_axis_full
+    prec = {"fp32": mtr.PREC_FP32, "bf16": mtr.PREC_BF16, "fp16": mtr.PREC_FP16}[name]
+    with load_ctx(mtr, pb, prec) as c:
+        out = c.mace_solve(M, S, pb.lam, pb.rho)
+    flat = out.reshape(-1)
+    planes = out.reshape(pb.vol.Z, -1).sum(axis=1, dtype=np.float64)
+    norm2 = float(np.dot(flat.astype(np.float64), flat.astype(np.float64)))
+    e = orc.rel_l2(flat[g["idx"]], g["f64"])
+    ep = float(np.abs(planes - g["f64_planes"]).max() / np.abs(g["f64_planes"]).max())
+    en = abs(norm2 - float(g["f64_norm2"])) / float(g["f64_norm2"])
+    say(f"three-axis full size, {name}: vs FP64 oracle sampled rel_l2 {e:.3e}, plane sums {ep:.3e}, "
+        f"norm2 {en:.3e} (bound {SPLIT_BOUND[name]:.0e})")
+    assert np.isfinite(flat).all()
+    assert e < SPLIT_BOUND[name] and ep < SPLIT_BOUND[name] and en < SPLIT_BOUND[name]
