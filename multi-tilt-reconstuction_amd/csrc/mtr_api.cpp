@@ -491,14 +491,14 @@ int fp_seg_planes()
 }
 
 // Device bytes of the streaming FP's Z segments for an agent of P pixels
-// and A views over nz planes, should it take that kernel: the partial ray
-// sums and the block order.
+// and A views over nz planes, should it take that kernel: the block order
+// of each segment (column groups of at least 4 columns) and the partial ray
+// sums of a segmented box.
 size_t fp_seg_bytes(size_t P, const mtr_projection_size &prj, int nz)
 {
     const int len = fp_seg_planes();
-    if (len <= 0 || nz < 2 * len) return 0;
-    const size_t nseg = (size_t)((nz + len - 1) / len);
-    return 4 * P * nseg + 4 * nseg * (size_t)prj.AngN * (size_t)((prj.X + 7) / 8);
+    const size_t nseg = len > 0 && nz >= 2 * len ? (size_t)((nz + len - 1) / len) : 1;
+    return 4 * nseg * (size_t)prj.AngN * (size_t)((prj.X + 3) / 4) + (nseg > 1 ? 4 * P * nseg : 0);
 }
 
 // Upper bound of one agent's device bytes over the kernel choices: geometry
