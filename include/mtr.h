@@ -149,6 +149,10 @@ int mtr_agent_count(const mtr_ctx *ctx, int *count);
 /* Upper bound of the HBM bytes an agent of this shape occupies at the
  * context precision (whatever kernels its geometry selects). */
 int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size_t *bytes);
+/* Device bytes an added agent holds now, and the most it held (staging
+ * buffers of the upload included); mtr_agent_footprint bounds both.  No
+ * counterpart in the reference. */
+int mtr_agent_device_bytes(const mtr_ctx *ctx, int agent, size_t *resident, size_t *peak);
 /* Kernels the geometry of an agent selected (no counterpart in the
  * reference, which has one kernel per operator):
  *   forward projection  MTR_FPK_GENERAL  ray-driven gather over the planes

@@ -172,6 +172,12 @@ class Context:
         _check(lib.mtr_agent_footprint(self._h, _ct.byref(prj), _ct.byref(b)), "agent_footprint")
         return b.value
 
+    def agent_device_bytes(self, agent: int) -> tuple[int, int]:
+        """(bytes held now, most bytes held while the agent was added)."""
+        r, pk = _ct.c_size_t(), _ct.c_size_t()
+        _check(lib.mtr_agent_device_bytes(self._h, agent, _ct.byref(r), _ct.byref(pk)), "agent_device_bytes")
+        return r.value, pk.value
+
     def agent_kernels(self, agent: int) -> tuple[int, int]:
         """(forward, back) projection kernels chosen for an agent: FPK_* and BPK_*."""
         f, b = _ct.c_int(), _ct.c_int()

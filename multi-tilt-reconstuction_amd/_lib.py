@@ -58,6 +58,7 @@ _PROTOS = {
     "mtr_agent_add_device": (_i, [_v, _pprj, _pd, _pd, _v, _pi]),
     "mtr_agent_count": (_i, [_v, _pi]),
     "mtr_agent_footprint": (_i, [_v, _pprj, ct.POINTER(_sz)]),
+    "mtr_agent_device_bytes": (_i, [_v, _i, ct.POINTER(_sz), ct.POINTER(_sz)]),
     "mtr_agent_kernels": (_i, [_v, _i, ct.POINTER(_i), ct.POINTER(_i)]),
     "mtr_bpt": (_i, [_v, _i, _pf]),
     "mtr_sirt": (_i, [_v, _i, _pf, _i, _f, _pf]),

@@ -341,6 +341,8 @@ struct Agent {
     float *d_seg = nullptr;          // per-segment partial ray sums [fp_seg][P] (fp_seg > 1)
     int *d_seg_ord = nullptr;        // per segment, (view, column group) pairs in crossing order
     int npartial = 0;      // per-block partials written by the last FP launch
+    size_t dev_bytes = 0;  // device bytes the agent holds
+    size_t dev_peak = 0;   // and at most held while it was added (staging included)
     int fast = 0;          // contiguous axis of the working volume (0 = X, canonical)
     int64_t st[3] = {0, 0, 0};
     void *d_tmp = nullptr; // canonical copy of the working volume (fast != 0)
@@ -985,6 +987,8 @@ int plain_p(mtr_ctx *ctx, Agent &ag, const float **out)
     }
     if (!ag.d_p_std) {
         HIP_TRY(hipMalloc(&ag.d_p_std, sizeof(float) * ag.P()));
+        ag.dev_bytes += sizeof(float) * ag.P();
+        ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes);
         const int64_t ss[3] = {ag.prj.Y, 1, (int64_t)ag.prj.X * ag.prj.Y};
         const int64_t ds[3] = {1, ag.prj.X, (int64_t)ag.prj.X * ag.prj.Y};
         HIP_TRY(launch_permute(MTR_PREC_FP32, ag.d_p, ag.d_p_std, ag.prj.X, ag.prj.Y, ag.prj.AngN, ss, ds,
@@ -1247,6 +1251,8 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         if (e_ != hipSuccess)                                                             \
             return undo(fail(MTR_ERR_OOM, "hipMalloc %zu bytes: %s", (size_t)(bytes),     \
                              hipGetErrorString(e_)));                                     \
+        ag.dev_bytes += (size_t)(bytes);                                                  \
+        ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes);                                \
     } while (0)
     ALLOC(ag.d_coef, sizeof(ViewCoef) * A);
     ALLOC(ag.d_plan, sizeof(FpPlan) * A);
@@ -1421,6 +1427,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
             if (p_host) {
                 if (hipMalloc(&tmp, sizeof(float) * P) != hipSuccess)
                     return undo(fail(MTR_ERR_OOM, "staging buffer"));
+                ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes + sizeof(float) * P);
                 if (hipMemcpyAsync(tmp, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
                     cleanup();
                     return undo(fail(MTR_ERR_HIP, "projection upload failed"));
@@ -1434,6 +1441,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                         cleanup();
                         return undo(fail(MTR_ERR_OOM, "staging buffer"));
                     }
+                    ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes + (tmp ? sizeof(float) * P : 0) + es * P);
                     e = launch_to_storage(ctx->precision, P, src, conv, ctx->stream);
                 }
                 const int64_t ss[3] = {1, prj->X, (int64_t)prj->X * prj->Y};
@@ -1757,6 +1765,15 @@ int mtr_agent_footprint(const mtr_ctx *ctx, const mtr_projection_size *prj, size
     if (!ctx->vol_set) return fail(MTR_ERR_STATE, "volume not set");
     if (!prj_valid(prj)) return fail(MTR_ERR_SHAPE, "invalid projection size");
     *bytes = agent_bytes(ctx->precision, ctx->vol, *prj);
+    return MTR_OK;
+}
+
+int mtr_agent_device_bytes(const mtr_ctx *ctx, int agent, size_t *resident, size_t *peak)
+{
+    MTR_TRY(check_agent(const_cast<mtr_ctx *>(ctx), agent));
+    if (!resident || !peak) return fail(MTR_ERR_ARG, "null argument");
+    *resident = ctx->agents[agent].dev_bytes;
+    *peak = ctx->agents[agent].dev_peak;
     return MTR_OK;
 }
 
