@@ -116,6 +116,26 @@ __device__ __forceinline__ double coord(const double *c, int X, int Y, int Z)
 // Index of the calling wave in its block, as a wave-uniform (scalar) value.
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
 
+// floor(g) as an int in one instruction (v_cvt_flr_i32_f32).
+__device__ __forceinline__ int cvt_flr(float g)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(g));
+    return r;
+}
+
+// One FP32 tap of a staged detector row: the fraction du = g - floor(g)
+// (v_fract_f32, which returns 0x1.fffffep-1 rather than 1.0 for g in
+// (-2^-25, 0)) and floor(g) + ib clamped to [0, lmax], as floor(g) clamped
+// to [-ib, lmax - ib] through one med3 of g against glo = -ib and
+// ghi = lmax - ib + 0.5 (no rounding: only comparisons), returned without
+// the + ib, which the caller folds into its row base.
+__device__ __forceinline__ int tap_clamped(float g, float glo, float ghi, float &du)
+{
+    du = __builtin_amdgcn_fractf(g);
+    return cvt_flr(__builtin_amdgcn_fmed3f(g, glo, ghi));
+}
+
 // Tent weight max(0, 1 - |t|) of a signed distance t (in detector
 // columns), formed in FP64 and rounded once, so small weights keep their
 // relative precision.
@@ -2107,9 +2127,10 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
         const double fb = floor(ub);
         const float f0 = (float)(ub - fb), step = (float)uc[3];
         const int ib = (int)fb - cfi0;
-        const float2 *row0 = lds_raw + (a & 1) * kColElems + lr;
-        const float2 *row1 = row0 + pitch;
         const int lmax = pitch - 1;
+        const float2 *row0 = lds_raw + (a & 1) * kColElems + lr + ib;
+        const float2 *row1 = row0 + pitch;
+        const float glo = (float)(-ib), ghi = (float)(lmax - ib) + 0.5f;
         // tap t of the run is plane Z0 + t = Z32 + OFF + t (OFF = 0, or 16
         // for the upper half of a 32-plane group with runs of 16)
         auto taps = [&](auto off_c) {
@@ -2120,9 +2141,8 @@ __global__ __launch_bounds__(64 * kColBpWaves) void k_bp_col(const AgentDev *__r
 #pragma unroll
                 for (int t = 0; t < RW; t++) {
                     const float g = fmaf((float)(OFF + t), step, f0);
-                    const float fg = floorf(g);
-                    const float du = g - fg;
-                    const int li = min(max(ib + (int)fg, 0), lmax);
+                    float du;
+                    const int li = tap_clamped(g, glo, ghi, du);
                     const float2 qa = row0[li], qb = row1[li];
                     S[t] = fmaf(wk0, fmaf(du, qa.y, qa.x), S[t]);
                     S[t] = fmaf(wk1, fmaf(du, qb.y, qb.x), S[t]);
@@ -2310,19 +2330,17 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         const double fb = floor(ub);
         const float f0 = (float)(ub - fb), step = (float)uc[3];
         const int ib = (int)fb - cfi0;
-        const float4 *stg = lds4 + (a & 1) * kCsMaxW;
         const int lmax = min(cfi1 - cfi0, kCsMaxW - 1);
-        const float ibf = (float)ib, lmaxf = (float)lmax;
+        const float4 *stg = lds4 + (a & 1) * kCsMaxW + ib;
+        const float glo = (float)(-ib), ghi = (float)(lmax - ib) + 0.5f;
         auto taps = [&](auto off_c) {
             constexpr int OFF = decltype(off_c)::value;
             if constexpr (!NORM) {
 #pragma unroll
                 for (int t = 0; t < RW; t++) {
                     const float g = fmaf((float)(OFF + t), step, f0);
-                    const float fg = floorf(g);
-                    const float du = g - fg;
-                    // clamp in FP32 on exact integers: one med3
-                    const int li = (int)__builtin_amdgcn_fmed3f(fg + ibf, 0.0f, lmaxf);
+                    float du;
+                    const int li = tap_clamped(g, glo, ghi, du);
                     const float4 q = stg[li];
                     S[t] = fmaf(wk0, fmaf(du, q.y, q.x), S[t]);
                     S[t] = fmaf(wk1, fmaf(du, q.w, q.z), S[t]);
