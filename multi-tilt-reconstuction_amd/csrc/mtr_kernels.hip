@@ -1754,8 +1754,11 @@ __device__ __forceinline__ void pfx_eval(ColRows &rw, const double2 *m, int nz, 
         double X2[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const double sg = r == 0 ? s0 : s0 + inv;
-            const int b = (int)__builtin_amdgcn_fmed3f(ceilf((float)sg), 0.0f, (float)nz);
+            // X2(s) = s P0(b) - P1(b) for any b in [ceil(s), floor(s) + 1]
+            // (a plane at z = s has weight 0): b = floor(s) + 1 from the
+            // FP64 s, clamped to [1, Z]; s < 0 is taken as 0 (X2 = 0)
+            const double sg = fmax(r == 0 ? s0 : s0 + inv, 0.0);
+            const int b = min((int)sg, nz - 1) + 1;
             double2 mm;
             if constexpr (UNIT) {
                 mm = make_double2((double)b, 0.5 * (double)b * (double)(b - 1));
