@@ -154,6 +154,21 @@ int make_plan(const double *xc, const double *yc, FpPlan &pl)
     return MTR_OK;
 }
 
+// Column-plan forward projection: the eight waves of a block take eight
+// views, one each, and meet at a barrier per round of columns; a view costs
+// about the pixels a column reaches in it, |xc3| (Z - 1) + 3.  Views sorted
+// by that count (ties in view order) make each block's views alike, so no
+// wave waits long at the barrier for a costlier one.  pfx_view[g] is the
+// view of sorted slot g (the sums of every view are unchanged).
+void pfx_view_order(const double *xc, int A, int nz, std::vector<FpPlan> &plan)
+{
+    std::vector<int> ord(A);
+    for (int v = 0; v < A; v++) ord[v] = v;
+    auto npix = [&](int v) { return (int)(std::fabs(xc[4 * v + 3]) * (double)(nz - 1)) + 3; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return npix(a) > npix(b); });
+    for (int g = 0; g < A; g++) plan[g].pfx_view = ord[g];
+}
+
 // Back-projection mode of an agent: the run-along-W kernel applies when
 // every view is row-separable along the same axis W, u does not depend on
 // W, and the tile footprint fits the fixed LDS pitch.
@@ -1166,6 +1181,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
         if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
     }
+    pfx_view_order(xc, A, ctx->vol.Z, plan);
     ctx->agents.emplace_back();
     Agent &ag = ctx->agents.back();
     auto undo = [&](int rc) {
@@ -1496,6 +1512,7 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
         }
         int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
         if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
+        plan[v].pfx_view = v;
         all_wx = all_wx && plan[v].kind == 1 && plan[v].W == 0;
     }
     t.lanes_k = all_wx;
@@ -2425,6 +2442,7 @@ int mtr_op_fp_noalloc(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_vo
             coef[v].c[4 + m] = yc[4 * v + m];
         }
         const int rc = make_plan(xc + 4 * v, yc + 4 * v, plan[v]);
+        plan[v].pfx_view = v;
         if (rc != MTR_OK) return fail(rc, "view %d: degenerate or non-finite geometry", v);
         lanes_k = lanes_k && plan[v].kind == 1 && plan[v].W == 0;
     }

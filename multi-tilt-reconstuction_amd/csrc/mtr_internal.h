@@ -33,7 +33,7 @@ struct FpPlan {
     int W, L, M;   // separable: plane axis, line (drive) axis, cross axis
     int wsign;     // separable: bW (+1/-1)
     int ncand;     // separable: lattice points per line, floor(2 Em) + 1
-    int pad;
+    int pfx_view;  // column-plan FP: the view that wave slot v of a block group takes
     double woff;   // separable: W = (k + 1 - yc0) * wsign
     // general: crossing point (p*, q*) = p0 + i*pi + k*pk + d*pd
     double p0, pi, pk, pd;

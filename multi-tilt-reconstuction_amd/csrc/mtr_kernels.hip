@@ -1807,8 +1807,11 @@ __global__ __launch_bounds__(64 * kPfxWaves) void k_fp_pfx(const AgentDev *__res
     const int s = sx * 8 + (blockIdx.x & 7);
     if (s >= ag.nstrip) return;
     const int lane = threadIdx.x & 63, wid = wave_id();
-    const int a = (idx - sx * nvg) * kPfxWaves + wid;
-    const bool view = a < ag.A;
+    // wave slot g of the strip's view groups takes the view the host put
+    // there (views of alike cost share a block), read through the scalar cache
+    const int g = (idx - sx * nvg) * kPfxWaves + wid;
+    const bool view = g < ag.A;
+    const int a = view ? ((const __attribute__((address_space(4))) FpPlan *)ag.plan)[g].pfx_view : 0;
     if (UNIT && !view) return;  // no block-wide barrier without staged moments
     const int nu = ag.nu, nv = ag.nv, L = nu + 2, nz = ag.Z, zs = ag.Zs;
     const int nslot = pfx_slots(WIN ? ag.col_win : 0, L);
