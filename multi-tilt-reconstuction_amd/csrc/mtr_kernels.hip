@@ -1596,7 +1596,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_fp_col(const AgentDev *__res
 // lanes l - 1 and l - 2 hand theirs over (DPP shifts), and lanes 2..63 own
 // pixels p .. p + 61.  The moments are formed in FP64 (the second
 // difference cancels about 16 of its 53 bits) once per column and block,
-// shared by the block's eight views (one per wave); t is rounded once to
+// shared by the block's eight views (one per wave; the host orders the views
+// by the pixels a column reaches, so a block's views cost alike); t is rounded once to
 // FP32 and added, weighted (1 - dv, dv), into the wave's two detector rows
 // in the strip's fixed column order, with the row window of ColRows.  An
 // untilted view (c3 = 0) puts every plane at ub: (1 - du) P0(Z) and
