@@ -347,7 +347,7 @@ def serving(mtr, args):
         "last_arrival_s": float(stream[-1][0]),
         "steps": steps,
         "mean_active_per_step": active_sum / max(steps, 1),
-        "gpu_step_fraction": busy / end,
+        "step_call_fraction": busy / end,
         "hbm_budget_gb": args.hbm_budget_gb,
         "max_active": args.max_active,
         "admission": args.admit if args.admit == "arrival" else f"smallest, max wait {args.admit_max_wait:g} s",

@@ -317,11 +317,15 @@ int mtr_sync(mtr_ctx *ctx);
  * max_active are active.  The worker prepares the requests the policy
  * takes next (device memory, geometry tables, uploads; their bytes count
  * as in use from then on) while ticks run, and admission forms the
- * BPT-sum initial model.  Server calls are for one host thread.  Each mtr_server_step runs one MACE
- * step of every active request, the SIRT iterations of all their agents
- * batched into shared launches, and retires the requests that finished
- * (their w-bar is kept on the host and their device memory freed).  A
- * request's result does not depend on what else shares its launches.
+ * BPT-sum initial model.  Server calls are for one host thread.  Each mtr_server_step first
+ * completes the previous step (waits for its device work and retires the
+ * requests that finished with it: their w-bar is kept on the host and their
+ * device memory freed), then admits, then launches one MACE step of every
+ * active request, the SIRT iterations of all their agents batched into
+ * shared launches, and returns while that step runs, so the caller's work
+ * between steps overlaps the GPU's.  A request whose last MACE step was
+ * launched is retired by the next mtr_server_step.  A request's result
+ * does not depend on what else shares its launches.
  * Times are seconds since mtr_server_create. */
 typedef struct mtr_server mtr_server;
 int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out);
@@ -344,8 +348,8 @@ int mtr_server_footprint(const mtr_server *srv, const mtr_volume_size *vol, int 
 int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const mtr_projection_size *prj,
                       const double *const *xc, const double *const *yc, const float *const *p, int M,
                       int S, float lambda, float rho, int64_t *request_id);
-/* Admission, one MACE step of the active requests, retirement; returns
- * after the step's device work has finished. */
+/* Completion of the previous step (retirement), admission, and the launch
+ * of one MACE step of the active requests; returns while it runs. */
 int mtr_server_step(mtr_server *srv, int *completed);
 /* done (and step's completed) count finished requests, including failed
  * ones (an admission that could not allocate, for instance). */
@@ -355,7 +359,8 @@ int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *don
  * request is pending or active; a failed request returns its error. */
 int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, double *times);
 int mtr_server_now(const mtr_server *srv, double *seconds);
-/* Per-kernel timing of the batched launches (MTR_K_*), as for a context. */
+/* Per-kernel timing of the batched launches (MTR_K_*), as for a context,
+ * over the completed steps. */
 int mtr_server_timing(mtr_server *srv, int on);
 int mtr_server_timing_get(mtr_server *srv, int kernel, int64_t *launches, double *ms);
 

@@ -2715,6 +2715,10 @@ struct mtr_server {
     AgentDev *d_tab = nullptr;
     int tab_cap = 0;
     std::vector<AgentDev> h_tab;
+    // A tick's launches are in flight until the next tick (or status call
+    // that needs them) synchronizes: the host's work between ticks
+    // (submissions, admission) overlaps the GPU's.
+    bool inflight = false;
     // Background preparation: a worker thread builds the contexts of the
     // requests the admission policy takes next (allocations, geometry
     // tables, projection uploads on prep_stream) while the GPU runs ticks;
@@ -3000,12 +3004,49 @@ int mtr_server_set_admission(mtr_server *srv, int policy, double max_wait_s)
 // on the wait), run one MACE step of every active request with their SIRT
 // iterations batched into shared launches, then retire the requests that
 // finished.
+namespace {
+
+// Completes the tick in flight: waits for its launches, hands the w-bar of
+// every request that reached its M steps to the host and frees its memory
+// (before the next tick is launched, so no free waits for GPU work).
+int finish_tick(mtr_server *srv, int *done)
+{
+    if (!srv->inflight) return MTR_OK;
+    HIP_TRY(hipStreamSynchronize(srv->eng->stream));
+    srv->inflight = false;
+    std::vector<ServerReq *> still;
+    for (ServerReq *r : srv->active) {
+        if (r->step < r->M) {
+            still.push_back(r);
+            continue;
+        }
+        r->result.resize((size_t)r->ctx->N);
+        const int rc = copy_volume_out(r->ctx, r->ctx->d_wbar, r->result.data());
+        r->t_done = srv->now();
+        if (rc != MTR_OK) {
+            r->error = g_err;
+            r->code = rc;
+            r->state = REQ_FAILED;
+        } else {
+            r->state = REQ_DONE;
+        }
+        release(srv, r);
+        (*done)++;
+        srv->n_done++;
+    }
+    srv->active.swap(still);
+    return collect_timing(srv->eng);
+}
+
+}  // namespace
+
 int mtr_server_step(mtr_server *srv, int *completed)
 {
     if (!srv) return fail(MTR_ERR_ARG, "null server");
     if (completed) *completed = 0;
     MTR_TRY(set_device(srv->eng));
     int done = 0;
+    MTR_TRY(finish_tick(srv, &done));
     while (!srv->pending.empty() && (int)srv->active.size() < srv->max_active) {
         // arrival order: the oldest request; smallest first: the oldest one
         // that has waited longer than max_wait, else the least work (the
@@ -3072,6 +3113,9 @@ int mtr_server_step(mtr_server *srv, int *completed)
         owner = std::move(own2);
     }
     const int n = (int)B.e.size();
+    // from here on this tick's work is on the stream (admissions above
+    // already launched their initial models)
+    srv->inflight = true;
     if (n > 0) {
         if (n > srv->tab_cap) {
             if (srv->d_tab) (void)hipFree(srv->d_tab);
@@ -3117,30 +3161,8 @@ int mtr_server_step(mtr_server *srv, int *completed)
         for (auto &ag : c->agents) MTR_TRY(from_canon(c, ag));
         r->step++;
     }
-    HIP_TRY(hipStreamSynchronize(srv->eng->stream));
-    std::vector<ServerReq *> still;
-    for (ServerReq *r : srv->active) {
-        if (r->step < r->M) {
-            still.push_back(r);
-            continue;
-        }
-        r->result.resize((size_t)r->ctx->N);
-        const int rc = copy_volume_out(r->ctx, r->ctx->d_wbar, r->result.data());
-        r->t_done = srv->now();
-        if (rc != MTR_OK) {
-            r->error = g_err;
-            r->code = rc;
-            r->state = REQ_FAILED;
-        } else {
-            r->state = REQ_DONE;
-        }
-        release(srv, r);
-        done++;
-        srv->n_done++;
-    }
-    srv->active.swap(still);
-    MTR_TRY(collect_timing(srv->eng));
-    // the next admissions' host work overlaps the next tick
+    // the next admissions' host work overlaps this tick (and so does the
+    // caller's, until the next step completes it)
     plan_prep(srv);
     if (completed) *completed = done;
     return MTR_OK;
