@@ -182,3 +182,28 @@ def test_background_preparation_takes_memory_at_submission(mtr, monkeypatch):
             res[pf] = [srv.result(r)[0] for r in ids]
     for a, b in zip(res["1"], res["0"]):
         assert np.array_equal(a, b)
+
+
+def test_a_step_returns_while_it_runs_and_the_next_retires(mtr):
+    # a tick returns with its launches in flight; the next step waits for
+    # them and retires the requests that finished, so a request of M steps
+    # is done after M + 1 step calls, and its result is ready only then
+    pb = req(mtr, 32, 28, 10, [0.0, 90.0], 71, M=2, S=2)
+    with mtr.Server(0) as srv:
+        rid = srv.submit_problem(pb)
+        assert srv.step() == 0 and srv.status()["active"] == 1
+        assert srv.step() == 0 and srv.status()["done"] == 0
+        with pytest.raises(mtr.MtrError):
+            srv.result(rid)  # the last step is still in flight
+        assert srv.step() == 1
+        st = srv.status()
+        assert st["done"] == 1 and st["active"] == 0 and st["hbm_in_use"] == 0
+        out, _ = srv.result(rid)
+        ts, ta, td = srv.times(rid)
+        assert ts <= ta <= td
+    with mtr.Context(0) as c:
+        c.set_volume(pb.vol)
+        for a in range(pb.K):
+            c.add_agent(pb.prjs[a], pb.xcs[a], pb.ycs[a], pb.ps[a])
+        ref = c.mace_solve(pb.M, pb.S, pb.lam, pb.rho)
+    assert np.array_equal(out, ref)
