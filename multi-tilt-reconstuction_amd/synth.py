@@ -174,25 +174,38 @@ SERVING = dict(n=64, rate=4.0, seed=5, xy=(256, 384, 512, 768), z=(64, 128, 256)
                lam=0.5, rho=0.5, n_ell=24)
 
 
+def serving_params(q: int, n: int = SERVING["n"], rate: float = SERVING["rate"], seed: int = SERVING["seed"],
+                   xy=SERVING["xy"], z=SERVING["z"], max_axes: int = SERVING["max_axes"]):
+    """(arrival seconds, xy, z, axis count) of request q of the seeded stream:
+    exponential inter-arrival gaps (-ln(1 - U) / rate) and uniform draws of
+    xy, z and the axis count from the generator's U(0,1) sequence of `seed`."""
+    u = uniform(seed, 4 * n)
+    t = 0.0
+    for k in range(q + 1):
+        t += -np.log1p(-u[4 * k]) / rate
+    s_xy = xy[min(int(u[4 * q + 1] * len(xy)), len(xy) - 1)]
+    s_z = z[min(int(u[4 * q + 2] * len(z)), len(z) - 1)]
+    K = 1 + min(int(u[4 * q + 3] * max_axes), max_axes - 1)
+    return t, s_xy, s_z, K
+
+
+def serving_request(q: int, n: int = SERVING["n"], rate: float = SERVING["rate"], seed: int = SERVING["seed"],
+                    xy=SERVING["xy"], z=SERVING["z"], max_axes: int = SERVING["max_axes"], M: int = SERVING["M"],
+                    S: int = SERVING["S"], n_ell: int = SERVING["n_ell"], theta=None, threads: int = 0):
+    """(arrival seconds, Problem) of request q of the seeded stream: K axes
+    evenly spaced over 180 degrees, seed 1000 seed + q."""
+    t, s_xy, s_z, K = serving_params(q, n, rate, seed, xy, z, max_axes)
+    phis = [180.0 * a / K for a in range(K)]
+    pb = make_problem(f"req{q}", s_xy, s_xy, s_z, n_ell, phis, seed=seed * 1000 + q, M=M, S=S,
+                      lam=SERVING["lam"], rho=SERVING["rho"], theta=theta, threads=threads)
+    return t, pb
+
+
 def serving_stream(n: int = SERVING["n"], rate: float = SERVING["rate"], seed: int = SERVING["seed"],
                    xy=SERVING["xy"], z=SERVING["z"], max_axes: int = SERVING["max_axes"], M: int = SERVING["M"],
                    S: int = SERVING["S"], n_ell: int = SERVING["n_ell"], theta=None, threads: int = 0):
-    """[(arrival seconds, Problem)] of a seeded request stream: exponential
-    inter-arrival gaps (-ln(1 - U) / rate) and uniform draws of xy, z and
-    the axis count from the generator's U(0,1) sequence of `seed`."""
-    u = uniform(seed, 4 * n)
-    t = 0.0
-    out = []
-    for q in range(n):
-        t += -np.log1p(-u[4 * q]) / rate
-        s_xy = xy[min(int(u[4 * q + 1] * len(xy)), len(xy) - 1)]
-        s_z = z[min(int(u[4 * q + 2] * len(z)), len(z) - 1)]
-        K = 1 + min(int(u[4 * q + 3] * max_axes), max_axes - 1)
-        phis = [180.0 * a / K for a in range(K)]
-        pb = make_problem(f"req{q}", s_xy, s_xy, s_z, n_ell, phis, seed=seed * 1000 + q, M=M, S=S,
-                          lam=SERVING["lam"], rho=SERVING["rho"], theta=theta, threads=threads)
-        out.append((t, pb))
-    return out
+    """[(arrival seconds, Problem)] of the seeded request stream (serving_request for q = 0 .. n - 1)."""
+    return [serving_request(q, n, rate, seed, xy, z, max_axes, M, S, n_ell, theta, threads) for q in range(n)]
 
 
 def config(name: str, threads: int = 0, **overrides) -> Problem:

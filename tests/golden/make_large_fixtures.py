@@ -44,6 +44,9 @@ CASES = {
     # the three-axis workload (axes 0/60/120, 1024x1024x256, 61 views) at one MACE step of one SIRT
     # iteration: the split-precision modes against the FP64 oracle at full size
     "c4_m1s1": ("c4", {}, 1, 1),
+    # request 33 of the seeded serving stream (256x256x64, axes 0/45/90/135, 61 views) with its whole
+    # schedule (10 MACE steps x 5 SIRT): the served form against the oracle at a serving size
+    "c5_req33": ("serve", {"q": 33}, None, None),
 }
 N_SAMPLE = 32768
 SAMPLE_SEED = 20261204
@@ -64,6 +67,9 @@ def sample_indices(N: int) -> np.ndarray:
 
 def problem(mtr, case: str):
     cfg, over, M, S = CASES[case]
+    if cfg == "serve":
+        pb = mtr.synth.serving_request(over["q"])[1]
+        return pb, pb.M, pb.S
     pb = mtr.synth.config(cfg, **over)
     return pb, M, S
 

@@ -191,6 +191,11 @@ def test_serving_stream_is_seeded_and_shaped(mtr):
         # axes evenly spaced over 180 degrees: the v gradient turns by 180/K
         ang = [np.degrees(np.arctan2(-yc[1], yc[2])) % 180.0 for yc in pb.ycs]
         assert np.allclose(np.diff(ang), 180.0 / pb.K) if pb.K > 1 else True
+    # one request of the stream on its own is the stream's request
+    for q in (0, 5, 11):
+        t, pb = mtr.synth.serving_request(q, n=12, xy=(16, 24), z=(8, 12), theta=[-30.0, 0.0, 30.0])
+        assert t == a[q][0] and pb.K == a[q][1].K and pb.vol.X == a[q][1].vol.X and pb.vol.Z == a[q][1].vol.Z
+        assert all(np.array_equal(x, y) for x, y in zip(pb.ps, a[q][1].ps))
 
 
 def test_server_without_device_is_an_error(mtr):

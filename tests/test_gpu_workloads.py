@@ -10,6 +10,8 @@ FunctionOnGPU.cu:212-244), in both oracle modes:
   dual-axis (0/90 deg) full schedule       256x256x64, M=10, S=5, against large_c2_sched_256.npz
   three-axis (0/60/120 deg) in FP32, BF16 and FP16 storage against the FP64 oracle,
       at 128x128x32 (M=2, S=2) and at full size (M=1, S=1, large_c4_m1s1.npz)
+  request 33 of the serving stream (256x256x64, four axes, M=10, S=5) served in a
+      mix of other stream requests, against large_c5_req33.npz
 
 Bars (relative L2 = ||gpu - oracle|| / ||oracle||): FP32 storage <= 1e-5
 against both oracle modes; BF16 <= 2e-2 and FP16 <= 3e-3 against FP64 (the
@@ -80,7 +82,7 @@ def test_four_axis_fast_solve_matches_both_oracles(mtr, orc):
     assert np.array_equal(out, again)
 
 
-def _against_fixture(mtr, orc, case):
+def _against_fixture(mtr, orc, case, solve=None):
     fx = fixtures()
     path = os.path.join(GOLD, f"large_{case}.npz")
     if not os.path.exists(path):
@@ -89,8 +91,11 @@ def _against_fixture(mtr, orc, case):
     pb, M, S = fx.problem(mtr, case)
     assert str(g["p_sha256"]) == fx.projections_hash(pb), "generator output differs from the fixture's inputs"
     say(f"{case}: {pb.vol.X}x{pb.vol.Y}x{pb.vol.Z}, K={pb.K}, M={M} S={S}")
-    with load_ctx(mtr, pb) as c:
-        out = c.mace_solve(M, S, pb.lam, pb.rho)
+    if solve is None:
+        with load_ctx(mtr, pb) as c:
+            out = c.mace_solve(M, S, pb.lam, pb.rho)
+    else:
+        out = solve(pb)
     flat = out.reshape(-1)
     idx = g["idx"]
     planes = out.reshape(pb.vol.Z, -1).sum(axis=1, dtype=np.float64)
@@ -113,6 +118,34 @@ def test_four_axis_full_size_one_step_matches_oracle(mtr, orc):
 
 def test_dual_axis_full_schedule_matches_oracle(mtr, orc):
     _against_fixture(mtr, orc, "c2_sched_256")
+
+
+def test_served_request_full_schedule_matches_oracle(mtr, orc):
+    # request 33 of the seeded serving stream (256x256x64, axes 0/45/90/135,
+    # 61 views, 10 MACE steps x 5 SIRT) served while requests 32, 34, 36 and
+    # 40 of the stream (1-4 axes, 64-256 planes) share its launches: against
+    # both oracle modes, and bit for bit against its standalone solve
+    def serve(pb):
+        others = [mtr.synth.serving_request(q)[1] for q in (32, 34, 36, 40)]
+        with mtr.Server(0) as srv:
+            srv.submit_problem(others[0])
+            srv.step()
+            rid = srv.submit_problem(pb)
+            srv.step()
+            for o in others[1:]:
+                srv.submit_problem(o)
+            steps = 0
+            while srv.status()["done"] < 5:
+                srv.step()
+                steps += 1
+                assert steps < 200
+            out, _ = srv.result(rid)
+        with load_ctx(mtr, pb) as c:
+            solo = c.mace_solve(pb.M, pb.S, pb.lam, pb.rho)
+        assert np.array_equal(out, solo)
+        return out
+
+    _against_fixture(mtr, orc, "c5_req33", serve)
 
 
 @pytest.fixture(scope="module")
