@@ -329,6 +329,7 @@ def serving(mtr, args):
         ts, ta, td = srv.times(r)
         lat.append(td - start - arr)
         wait.append(ta - start - arr)
+        end = max(end, td - start)  # the last w-bar reaches the host after the step that counted it done
     srv.close()
     lat = np.array(lat)
     return {

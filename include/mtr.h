@@ -318,14 +318,18 @@ int mtr_sync(mtr_ctx *ctx);
  * takes next (device memory, geometry tables, uploads; their bytes count
  * as in use from then on) while ticks run, and admission forms the
  * BPT-sum initial model.  Server calls are for one host thread.  Each mtr_server_step first
- * completes the previous step (waits for its device work and retires the
- * requests that finished with it: their w-bar is kept on the host and their
- * device memory freed), then admits, then launches one MACE step of every
- * active request, the SIRT iterations of all their agents batched into
- * shared launches, and returns while that step runs, so the caller's work
- * between steps overlaps the GPU's.  A request whose last MACE step was
- * launched is retired by the next mtr_server_step.  A request's result
- * does not depend on what else shares its launches.
+ * completes the previous step (waits for its device work; the requests
+ * that finished with it count as done and go to a second thread, the
+ * retirer, which copies their w-bar to the host and frees their device
+ * memory while the next steps run), then admits, then launches one MACE
+ * step of every active request, the SIRT iterations of all their agents
+ * batched into shared launches, and returns while that step runs, so the
+ * caller's work between steps overlaps the GPU's.  A request whose last
+ * MACE step was launched is done after the next mtr_server_step;
+ * mtr_server_result waits for its hand-over.  Request memory comes from a
+ * stream-ordered pool of the server's, so a retirement waits for no other
+ * device work.  A request's result does not depend on what else shares
+ * its launches.
  * Times are seconds since mtr_server_create. */
 typedef struct mtr_server mtr_server;
 int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out);
@@ -352,10 +356,12 @@ int mtr_server_submit(mtr_server *srv, const mtr_volume_size *vol, int K, const 
  * of one MACE step of the active requests; returns while it runs. */
 int mtr_server_step(mtr_server *srv, int *completed);
 /* done (and step's completed) count finished requests, including failed
- * ones (an admission that could not allocate, for instance). */
+ * ones (an admission that could not allocate, for instance).  hbm_in_use
+ * still counts a done request until the retirer has freed its memory. */
 int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *done, size_t *hbm_in_use);
 /* w-bar of a finished request ([Z][Y][X]); times[3] (nullable) receives
- * submission, admission and completion times.  MTR_ERR_STATE while the
+ * submission, admission and completion (w-bar on the host) times; waits
+ * while the retirer hands the request over.  MTR_ERR_STATE while the
  * request is pending or active; a failed request returns its error. */
 int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, double *times);
 int mtr_server_now(const mtr_server *srv, double *seconds);

@@ -400,6 +400,10 @@ struct mtr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false: the stream belongs to a server
+    // A server's requests take their device memory from the server's pool,
+    // ordered on the context's stream, so freeing it waits for no other
+    // work on the device (hipFree does); other contexts use hipMalloc.
+    hipMemPool_t pool = nullptr;
     int precision = MTR_PREC_FP32;
     mtr_volume_size vol{};
     bool vol_set = false;
@@ -442,6 +446,28 @@ struct mtr_ctx {
 };
 
 namespace {
+
+// Device memory a context owns (see mtr_ctx::pool).
+hipError_t dev_alloc(const mtr_ctx *ctx, void **p, size_t bytes)
+{
+    return ctx->pool ? hipMallocFromPoolAsync(p, bytes, ctx->pool, ctx->stream) : hipMalloc(p, bytes);
+}
+
+void dev_free(const mtr_ctx *ctx, void *p)
+{
+    if (!p) return;
+    if (ctx->pool)
+        (void)hipFreeAsync(p, ctx->stream);
+    else
+        (void)hipFree(p);
+}
+
+// A context's temporary, freed on scope exit.
+struct CtxBuf {
+    const mtr_ctx *ctx;
+    void *p = nullptr;
+    ~CtxBuf() { dev_free(ctx, p); }
+};
 
 int set_device(mtr_ctx *ctx)
 {
@@ -983,12 +1009,12 @@ int tune_all(mtr_ctx *ctx)
 int ensure_plain_scratch(mtr_ctx *ctx, int64_t P)
 {
     if (ctx->sc_cap >= P) return MTR_OK;
-    if (ctx->d_s) (void)hipFree(ctx->d_s);
-    if (ctx->d_cc) (void)hipFree(ctx->d_cc);
+    dev_free(ctx, ctx->d_s);
+    dev_free(ctx, ctx->d_cc);
     ctx->d_s = ctx->d_cc = nullptr;
     ctx->sc_cap = 0;
-    HIP_TRY(hipMalloc(&ctx->d_s, sizeof(float) * P));
-    HIP_TRY(hipMalloc(&ctx->d_cc, sizeof(float) * P));
+    HIP_TRY(dev_alloc(ctx, (void **)&ctx->d_s, sizeof(float) * P));
+    HIP_TRY(dev_alloc(ctx, (void **)&ctx->d_cc, sizeof(float) * P));
     ctx->sc_cap = P;
     return MTR_OK;
 }
@@ -1001,7 +1027,7 @@ int plain_p(mtr_ctx *ctx, Agent &ag, const float **out)
         return MTR_OK;
     }
     if (!ag.d_p_std) {
-        HIP_TRY(hipMalloc(&ag.d_p_std, sizeof(float) * ag.P()));
+        HIP_TRY(dev_alloc(ctx, (void **)&ag.d_p_std, sizeof(float) * ag.P()));
         ag.dev_bytes += sizeof(float) * ag.P();
         ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes);
         const int64_t ss[3] = {ag.prj.Y, 1, (int64_t)ag.prj.X * ag.prj.Y};
@@ -1190,7 +1216,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
                         (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
                         (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_seg, (void *)g.d_seg_ord})
-            if (q) (void)hipFree(q);
+            dev_free(ctx, q);
         ctx->agents.pop_back();
         return rc;
     };
@@ -1263,7 +1289,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
     const int64_t P = ag.P();
 #define ALLOC(ptr, bytes)                                                                 \
     do {                                                                                  \
-        hipError_t e_ = hipMalloc((void **)&(ptr), (bytes));                              \
+        hipError_t e_ = dev_alloc(ctx, (void **)&(ptr), (bytes));                         \
         if (e_ != hipSuccess)                                                             \
             return undo(fail(MTR_ERR_OOM, "hipMalloc %zu bytes: %s", (size_t)(bytes),     \
                              hipGetErrorString(e_)));                                     \
@@ -1437,11 +1463,11 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
             void *conv = nullptr;
             auto cleanup = [&]() {
                 (void)hipStreamSynchronize(ctx->stream);
-                if (tmp) (void)hipFree(tmp);
-                if (conv) (void)hipFree(conv);
+                dev_free(ctx, tmp);
+                dev_free(ctx, conv);
             };
             if (p_host) {
-                if (hipMalloc(&tmp, sizeof(float) * P) != hipSuccess)
+                if (dev_alloc(ctx, (void **)&tmp, sizeof(float) * P) != hipSuccess)
                     return undo(fail(MTR_ERR_OOM, "staging buffer"));
                 ag.dev_peak = std::max(ag.dev_peak, ag.dev_bytes + sizeof(float) * P);
                 if (hipMemcpyAsync(tmp, p_host, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
@@ -1453,7 +1479,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
             hipError_t e = hipSuccess;
             if (ag.rowfast) {
                 if (ctx->precision != MTR_PREC_FP32) {
-                    if (hipMalloc(&conv, es * P) != hipSuccess) {
+                    if (dev_alloc(ctx, &conv, es * P) != hipSuccess) {
                         cleanup();
                         return undo(fail(MTR_ERR_OOM, "staging buffer"));
                     }
@@ -1604,8 +1630,8 @@ int copy_volume_out(mtr_ctx *ctx, const void *dev, float *host)
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         return MTR_OK;
     }
-    DevBuf tmp;
-    HIP_TRY(hipMalloc(&tmp.p, sizeof(float) * ctx->N));
+    CtxBuf tmp{ctx};
+    HIP_TRY(dev_alloc(ctx, &tmp.p, sizeof(float) * ctx->N));
     HIP_TRY(launch_from_storage(ctx->precision, ctx->N, dev, (float *)tmp.p, ctx->stream));
     HIP_TRY(hipMemcpyAsync(host, tmp.p, sizeof(float) * ctx->N, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1619,8 +1645,8 @@ int copy_volume_in(mtr_ctx *ctx, void *dev, const float *host)
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         return MTR_OK;
     }
-    DevBuf tmp;
-    HIP_TRY(hipMalloc(&tmp.p, sizeof(float) * ctx->N));
+    CtxBuf tmp{ctx};
+    HIP_TRY(dev_alloc(ctx, &tmp.p, sizeof(float) * ctx->N));
     HIP_TRY(hipMemcpyAsync(tmp.p, host, sizeof(float) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(launch_to_storage(ctx->precision, ctx->N, (const float *)tmp.p, dev, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1633,11 +1659,11 @@ int scratch_init(int device);  // the allocation-free entry point's per-device s
 
 int alloc_tables(mtr_ctx *c)
 {
-    if (hipMalloc(&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&c->d_ags_fp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&c->d_ags_bp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
-        hipMalloc(&c->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
-        hipMalloc(&c->d_cons_scalar, sizeof(double)) != hipSuccess)
+    if (dev_alloc(c, (void **)&c->d_ags, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        dev_alloc(c, (void **)&c->d_ags_fp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        dev_alloc(c, (void **)&c->d_ags_bp, sizeof(AgentDev) * kMaxAgents) != hipSuccess ||
+        dev_alloc(c, (void **)&c->d_cons_partial, sizeof(double) * 8192) != hipSuccess ||
+        dev_alloc(c, (void **)&c->d_cons_scalar, sizeof(double)) != hipSuccess)
         return fail(MTR_ERR_OOM, "context tables");
     return MTR_OK;
 }
@@ -1686,11 +1712,11 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
                         (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_seg, (void *)ag.d_seg_ord})
-            if (q) (void)hipFree(q);
+            dev_free(ctx, q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,
                     (void *)ctx->d_cons_partial, (void *)ctx->d_cons_scalar, (void *)ctx->d_s, (void *)ctx->d_cc})
-        if (q) (void)hipFree(q);
+        dev_free(ctx, q);
     for (auto &pd : ctx->pending) {
         (void)hipEventDestroy(pd.a);
         (void)hipEventDestroy(pd.b);
@@ -1751,7 +1777,7 @@ int mtr_ctx_set_volume(mtr_ctx *ctx, const mtr_volume_size *vol)
     MTR_TRY(set_device(ctx));
     ctx->vol = *vol;
     ctx->N = (int64_t)vol->X * vol->Y * vol->Z;
-    HIP_TRY(hipMalloc(&ctx->d_wbar, elem_size(ctx->precision) * ctx->N));
+    HIP_TRY(dev_alloc(ctx, &ctx->d_wbar, elem_size(ctx->precision) * ctx->N));
     HIP_TRY(launch_fill(ctx->precision, ctx->N, ctx->d_wbar, 0.f, ctx->stream));
     ctx->vol_set = true;
     return MTR_OK;
@@ -2667,7 +2693,7 @@ int mtr_sync(mtr_ctx *ctx)
 
 namespace {
 
-enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3 };
+enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = 3, REQ_RETIRING = 4 };
 // host-side preparation of a pending request (its context, tables and
 // uploads) by the server's worker thread
 enum { PREP_NONE = 0, PREP_QUEUED = 1, PREP_RUNNING = 2, PREP_DONE = 3, PREP_FAILED = 4 };
@@ -2686,7 +2712,10 @@ struct ServerReq {
     size_t bytes = 0;
     double work = 0;  // voxel-view updates of the whole solve: (2SM + 1) N sum_a A_a
     mtr_ctx *ctx = nullptr;
-    std::vector<float> result;
+    // From REQ_RETIRING on, state, result and t_done are written under
+    // mtr_server::mu (by the retirer).
+    std::unique_ptr<float[]> result;  // w-bar of a finished request, result_n values
+    size_t result_n = 0;
     std::string error;
     int code = MTR_ERR_HIP;  // status of a failed request
     // PREP_*: written under mtr_server::mu (the tick thread sets QUEUED and
@@ -2706,7 +2735,7 @@ struct mtr_server {
     int admit_policy = MTR_ADMIT_ARRIVAL;
     double max_wait = 0;  // seconds, MTR_ADMIT_SMALLEST only
     mtr_ctx *eng = nullptr;  // engine context: the stream every request shares
-    size_t used = 0;
+    size_t used = 0;  // under mu
     int64_t next_id = 1;
     int n_done = 0;
     std::deque<ServerReq *> pending;
@@ -2715,10 +2744,23 @@ struct mtr_server {
     AgentDev *d_tab = nullptr;
     int tab_cap = 0;
     std::vector<AgentDev> h_tab;
-    // A tick's launches are in flight until the next tick (or status call
-    // that needs them) synchronizes: the host's work between ticks
-    // (submissions, admission) overlaps the GPU's.
+    // A tick's launches are in flight until the next step synchronizes:
+    // the host's work between ticks (submissions, admission) overlaps the
+    // GPU's.  The requests that finished with it go to a second thread, the
+    // retirer, which copies their w-bar to the host (on out_stream, through
+    // pinned staging: a copy into pageable memory runs at a fraction of the
+    // link's rate beside a tick's kernels) and frees their device memory
+    // while the next ticks run.
     bool inflight = false;
+    hipStream_t out_stream = nullptr;
+    hipMemPool_t pool = nullptr;  // the requests' device memory
+    float *h_stage = nullptr;  // the retirer's
+    size_t stage_cap = 0;
+    std::thread retirer;
+    std::deque<ServerReq *> ret_q;
+    int retiring = 0;  // handed to the retirer and not yet retired (under mu)
+    std::condition_variable cv_ret, cv_retired;
+    bool ret_quit = false;
     // Background preparation: a worker thread builds the contexts of the
     // requests the admission policy takes next (allocations, geometry
     // tables, projection uploads on prep_stream) while the GPU runs ticks;
@@ -2726,7 +2768,7 @@ struct mtr_server {
     bool prefetch = true;
     hipStream_t prep_stream = nullptr;
     std::thread worker;
-    std::mutex mu;
+    mutable std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServerReq *> prep_q;
     bool quit = false;
@@ -2751,6 +2793,7 @@ void release(mtr_server *srv, ServerReq *r)
         mtr_ctx_destroy(r->ctx);
         r->ctx = nullptr;
     }
+    std::lock_guard<std::mutex> lk(srv->mu);
     srv->used -= std::min(srv->used, r->bytes);
 }
 
@@ -2763,6 +2806,7 @@ int prepare(mtr_server *srv, ServerReq *r, hipStream_t st)
     c->device = srv->device;
     c->stream = st;
     c->own_stream = false;
+    c->pool = srv->pool;
     c->precision = srv->precision;
     c->joint_tune = false;  // its agents share launches with other requests' agents
     c->tune_timed = false;  // no timed tuning inside a tick: cached tiles or the heuristic ones
@@ -2868,6 +2912,68 @@ int take_prepared(mtr_server *srv, ServerReq *r)
     return prepare(srv, r, srv->eng->stream);
 }
 
+// Hands a finished request's w-bar to the host and frees its device
+// memory (on the retirer, or on the tick thread when there is none).
+void retire_one(mtr_server *srv, ServerReq *r)
+{
+    const size_t n = (size_t)r->ctx->N;
+    int rc = MTR_OK;
+    if (n > srv->stage_cap) {
+        if (srv->h_stage) (void)hipHostFree(srv->h_stage);
+        srv->h_stage = nullptr;
+        srv->stage_cap = 0;
+        if (hipHostMalloc((void **)&srv->h_stage, sizeof(float) * n, hipHostMallocDefault) == hipSuccess)
+            srv->stage_cap = n;
+        else
+            rc = fail(MTR_ERR_OOM, "pinned staging of %zu floats", n);
+    }
+    if (rc == MTR_OK) {
+        r->ctx->stream = srv->out_stream;  // its destruction then waits for this copy only
+        rc = copy_volume_out(r->ctx, r->ctx->d_wbar, srv->h_stage);
+    }
+    std::unique_ptr<float[]> res;
+    if (rc == MTR_OK) {
+        res.reset(new (std::nothrow) float[n]);
+        if (res)
+            std::memcpy(res.get(), srv->h_stage, sizeof(float) * n);
+        else
+            rc = fail(MTR_ERR_OOM, "host result of %zu floats", n);
+    }
+    const std::string err = rc != MTR_OK ? g_err : std::string();
+    // done once the w-bar is on the host: before the memory it frees can
+    // admit another request
+    const double t_done = srv->now();
+    release(srv, r);
+    std::lock_guard<std::mutex> lk(srv->mu);
+    r->result = std::move(res);
+    r->result_n = n;
+    r->t_done = t_done;
+    if (rc != MTR_OK) {
+        r->error = err;
+        r->code = rc;
+        r->state = REQ_FAILED;
+    } else {
+        r->state = REQ_DONE;
+    }
+    if (srv->retiring > 0) srv->retiring--;
+    srv->cv_retired.notify_all();
+}
+
+void retire_worker(mtr_server *srv)
+{
+    (void)hipSetDevice(srv->device);
+    std::unique_lock<std::mutex> lk(srv->mu);
+    for (;;) {
+        srv->cv_ret.wait(lk, [&] { return srv->ret_quit || !srv->ret_q.empty(); });
+        if (srv->ret_quit) return;  // the server releases what is left
+        ServerReq *r = srv->ret_q.front();
+        srv->ret_q.pop_front();
+        lk.unlock();
+        retire_one(srv, r);
+        lk.lock();
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -2889,9 +2995,38 @@ int mtr_server_create(int device, int precision, size_t hbm_budget, int max_acti
     srv->max_active = max_active;
     srv->eng = eng;
     srv->t0 = std::chrono::steady_clock::now();
+    if (hipStreamCreateWithFlags(&srv->out_stream, hipStreamNonBlocking) != hipSuccess) {
+        mtr_ctx_destroy(eng);
+        delete srv;
+        return fail(MTR_ERR_HIP, "server result stream");
+    }
+    {
+        // freed request memory stays in the pool (up to a quarter of the
+        // budget) for the next requests rather than going back to the
+        // driver at each synchronization
+        hipMemPoolProps pp = {};
+        pp.allocType = hipMemAllocationTypePinned;
+        pp.location.type = hipMemLocationTypeDevice;
+        pp.location.id = device;
+        uint64_t keep = hbm_budget / 4;
+        if (hipMemPoolCreate(&srv->pool, &pp) != hipSuccess ||
+            hipMemPoolSetAttribute(srv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            if (srv->pool) (void)hipMemPoolDestroy(srv->pool);
+            (void)hipStreamDestroy(srv->out_stream);
+            mtr_ctx_destroy(eng);
+            delete srv;
+            return fail(MTR_ERR_HIP, "server memory pool");
+        }
+    }
+    try {
+        srv->retirer = std::thread(retire_worker, srv);
+    } catch (...) {  // no thread to spare: retire inside the steps
+    }
     if (const char *pf = std::getenv("MTR_SERVER_PREFETCH")) srv->prefetch = std::atoi(pf) != 0;
     if (srv->prefetch) {
         if (hipStreamCreateWithFlags(&srv->prep_stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipMemPoolDestroy(srv->pool);
+            (void)hipStreamDestroy(srv->out_stream);
             mtr_ctx_destroy(eng);
             delete srv;
             return fail(MTR_ERR_HIP, "server preparation stream");
@@ -2920,9 +3055,23 @@ int mtr_server_destroy(mtr_server *srv)
         srv->cv_work.notify_all();
         srv->worker.join();
     }
+    if (srv->retirer.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            srv->ret_quit = true;
+        }
+        srv->cv_ret.notify_all();
+        srv->retirer.join();
+    }
     (void)hipStreamSynchronize(srv->eng->stream);
     for (auto &kv : srv->reqs) release(srv, kv.second.get());
+    // the frees just queued on the server's streams, then the pool
+    for (hipStream_t st : {srv->eng->stream, srv->out_stream, srv->prep_stream})
+        if (st) (void)hipStreamSynchronize(st);
+    if (srv->pool) (void)hipMemPoolDestroy(srv->pool);
     if (srv->prep_stream) (void)hipStreamDestroy(srv->prep_stream);
+    if (srv->out_stream) (void)hipStreamDestroy(srv->out_stream);
+    if (srv->h_stage) (void)hipHostFree(srv->h_stage);
     if (srv->d_tab) (void)hipFree(srv->d_tab);
     mtr_ctx_destroy(srv->eng);
     delete srv;
@@ -2999,47 +3148,44 @@ int mtr_server_set_admission(mtr_server *srv, int policy, double max_wait_s)
     return MTR_OK;
 }
 
-// One scheduling tick: admit pending requests while the HBM budget and the
-// slot count allow (in arrival order, or smallest solve first with a bound
-// on the wait), run one MACE step of every active request with their SIRT
-// iterations batched into shared launches, then retire the requests that
-// finished.
 namespace {
 
-// Completes the tick in flight: waits for its launches, hands the w-bar of
-// every request that reached its M steps to the host and frees its memory
-// (before the next tick is launched, so no free waits for GPU work).
+// Completes the tick in flight: waits for its launches; the requests that
+// reached their M steps leave the active set (their slots are free for
+// this step's admissions) and count as done; the retirer takes them over.
 int finish_tick(mtr_server *srv, int *done)
 {
     if (!srv->inflight) return MTR_OK;
     HIP_TRY(hipStreamSynchronize(srv->eng->stream));
     srv->inflight = false;
-    std::vector<ServerReq *> still;
-    for (ServerReq *r : srv->active) {
-        if (r->step < r->M) {
-            still.push_back(r);
-            continue;
-        }
-        r->result.resize((size_t)r->ctx->N);
-        const int rc = copy_volume_out(r->ctx, r->ctx->d_wbar, r->result.data());
-        r->t_done = srv->now();
-        if (rc != MTR_OK) {
-            r->error = g_err;
-            r->code = rc;
-            r->state = REQ_FAILED;
-        } else {
-            r->state = REQ_DONE;
-        }
-        release(srv, r);
-        (*done)++;
-        srv->n_done++;
-    }
+    std::vector<ServerReq *> still, fin;
+    for (ServerReq *r : srv->active) (r->step < r->M ? still : fin).push_back(r);
     srv->active.swap(still);
+    *done += (int)fin.size();
+    srv->n_done += (int)fin.size();
+    if (srv->retirer.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(srv->mu);
+            for (ServerReq *r : fin) {
+                r->state = REQ_RETIRING;
+                srv->ret_q.push_back(r);
+            }
+            srv->retiring += (int)fin.size();
+        }
+        if (!fin.empty()) srv->cv_ret.notify_one();
+    } else {
+        for (ServerReq *r : fin) retire_one(srv, r);
+    }
     return collect_timing(srv->eng);
 }
 
 }  // namespace
 
+// One scheduling tick: complete the previous one (its finished requests go
+// to the retirer), admit pending requests while the HBM budget and the
+// slot count allow (in arrival order, or smallest solve first with a bound
+// on the wait), and launch one MACE step of every active request with their
+// SIRT iterations batched into shared launches.
 int mtr_server_step(mtr_server *srv, int *completed)
 {
     if (!srv) return fail(MTR_ERR_ARG, "null server");
@@ -3058,14 +3204,17 @@ int mtr_server_step(mtr_server *srv, int *completed)
                 if ((*it)->work < (*pick)->work) pick = it;
         ServerReq *r = *pick;
         // a request queued for preparation already counts in `used`
-        size_t need = 0;
         {
-            std::lock_guard<std::mutex> lk(srv->mu);
-            need = r->prep == PREP_NONE ? r->bytes : 0;
+            std::unique_lock<std::mutex> lk(srv->mu);
+            const size_t need = r->prep == PREP_NONE ? r->bytes : 0;
+            // with nothing else to launch, wait for memory the retirer is
+            // about to free rather than return an empty tick
+            if (srv->active.empty())
+                srv->cv_retired.wait(lk, [&] { return srv->used + need <= srv->budget || srv->retiring == 0; });
+            if (srv->used + need > srv->budget) break;
+            srv->used += need;
         }
-        if (srv->used + need > srv->budget) break;
         srv->pending.erase(pick);
-        srv->used += need;
         r->t_admit = srv->now();
         int rc = take_prepared(srv, r);
         if (rc == MTR_OK) rc = activate(srv, r);
@@ -3174,7 +3323,10 @@ int mtr_server_status(const mtr_server *srv, int *pending, int *active, int *don
     if (pending) *pending = (int)srv->pending.size();
     if (active) *active = (int)srv->active.size();
     if (done) *done = srv->n_done;
-    if (hbm_in_use) *hbm_in_use = srv->used;
+    if (hbm_in_use) {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        *hbm_in_use = srv->used;
+    }
     return MTR_OK;
 }
 
@@ -3184,6 +3336,10 @@ int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, dou
     auto it = srv->reqs.find(request_id);
     if (it == srv->reqs.end()) return fail(MTR_ERR_RANGE, "unknown request %lld", (long long)request_id);
     ServerReq *r = it->second.get();
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        srv->cv_retired.wait(lk, [&] { return r->state != REQ_RETIRING; });
+    }
     if (times) {
         times[0] = r->t_submit;
         times[1] = r->t_admit;
@@ -3191,7 +3347,7 @@ int mtr_server_result(mtr_server *srv, int64_t request_id, float *wbar_host, dou
     }
     if (r->state == REQ_FAILED) return fail(r->code, "request %lld failed: %s", (long long)request_id, r->error.c_str());
     if (r->state != REQ_DONE) return fail(MTR_ERR_STATE, "request %lld not finished", (long long)request_id);
-    if (wbar_host) std::memcpy(wbar_host, r->result.data(), sizeof(float) * r->result.size());
+    if (wbar_host) std::memcpy(wbar_host, r->result.get(), sizeof(float) * r->result_n);
     return MTR_OK;
 }
 

@@ -178,16 +178,17 @@ def test_background_preparation_takes_memory_at_submission(mtr, monkeypatch):
             srv.step()
             ids += [srv.submit_problem(p) for p in pbs[1:]]
             drain(srv, 3)
+            res[pf] = [srv.result(r)[0] for r in ids]  # waits for each hand-over
             assert srv.status()["hbm_in_use"] == 0
-            res[pf] = [srv.result(r)[0] for r in ids]
     for a, b in zip(res["1"], res["0"]):
         assert np.array_equal(a, b)
 
 
 def test_a_step_returns_while_it_runs_and_the_next_retires(mtr):
     # a tick returns with its launches in flight; the next step waits for
-    # them and retires the requests that finished, so a request of M steps
-    # is done after M + 1 step calls, and its result is ready only then
+    # them and hands the requests that finished to the retirer, so a request
+    # of M steps is done after M + 1 step calls, and its result is ready
+    # only then (the result call waits for the retirer's copy)
     pb = req(mtr, 32, 28, 10, [0.0, 90.0], 71, M=2, S=2)
     with mtr.Server(0) as srv:
         rid = srv.submit_problem(pb)
@@ -197,8 +198,9 @@ def test_a_step_returns_while_it_runs_and_the_next_retires(mtr):
             srv.result(rid)  # the last step is still in flight
         assert srv.step() == 1
         st = srv.status()
-        assert st["done"] == 1 and st["active"] == 0 and st["hbm_in_use"] == 0
+        assert st["done"] == 1 and st["active"] == 0
         out, _ = srv.result(rid)
+        assert srv.status()["hbm_in_use"] == 0  # retired: its device memory is back
         ts, ta, td = srv.times(rid)
         assert ts <= ta <= td
     with mtr.Context(0) as c:
