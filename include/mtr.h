@@ -307,14 +307,15 @@ int mtr_sync(mtr_ctx *ctx);
 
 /* ---- request server (continuous batching) ----------------------------
  * No counterpart in the reference, which runs one reconstruction per MPI
- * job (MultiAxis.cpp:101-230).  A server owns one device stream, and a
- * worker thread with a stream of its own.  A
+ * job (MultiAxis.cpp:101-230).  A server owns one device stream, four
+ * preparation workers (threads, each with a stream of its own) and a
+ * retirer thread with a stream of its own.  A
  * request is a volume box and K tilt series (each with its own detector
  * size and geometry) plus its solve parameters (M MACE steps of S SIRT
  * iterations, lambda, rho); the server copies it on submission.  Requests
  * are admitted (in arrival order unless mtr_server_set_admission says
  * otherwise) while their HBM footprint fits the budget and fewer than
- * max_active are active.  The worker prepares the requests the policy
+ * max_active are active.  The workers prepare the requests the policy
  * takes next (device memory, geometry tables, uploads; their bytes count
  * as in use from then on) while ticks run, and admission forms the
  * BPT-sum initial model.  Server calls are for one host thread.  Each mtr_server_step first

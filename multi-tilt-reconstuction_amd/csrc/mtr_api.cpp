@@ -2727,6 +2727,8 @@ struct ServerReq {
 
 }  // namespace
 
+constexpr int kPrepWorkers = 4;  // request preparation threads of a server
+
 struct mtr_server {
     int device = 0;
     int precision = MTR_PREC_FP32;
@@ -2761,13 +2763,14 @@ struct mtr_server {
     int retiring = 0;  // handed to the retirer and not yet retired (under mu)
     std::condition_variable cv_ret, cv_retired;
     bool ret_quit = false;
-    // Background preparation: a worker thread builds the contexts of the
+    // Background preparation: worker threads build the contexts of the
     // requests the admission policy takes next (allocations, geometry
-    // tables, projection uploads on prep_stream) while the GPU runs ticks;
-    // admission then only launches their normalisers and initial models.
+    // tables, projection uploads, each worker on a stream of its own) while
+    // the GPU runs ticks; admission then only launches their normalisers
+    // and initial models.
     bool prefetch = true;
-    hipStream_t prep_stream = nullptr;
-    std::thread worker;
+    std::vector<hipStream_t> prep_streams;
+    std::vector<std::thread> workers;
     mutable std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<ServerReq *> prep_q;
@@ -2835,7 +2838,7 @@ int activate(mtr_server *srv, ServerReq *r)
     return MTR_OK;
 }
 
-void prep_worker(mtr_server *srv)
+void prep_worker(mtr_server *srv, hipStream_t st)
 {
     (void)hipSetDevice(srv->device);
     std::unique_lock<std::mutex> lk(srv->mu);
@@ -2846,7 +2849,7 @@ void prep_worker(mtr_server *srv)
         srv->prep_q.pop_front();
         r->prep = PREP_RUNNING;
         lk.unlock();
-        const int rc = prepare(srv, r, srv->prep_stream);
+        const int rc = prepare(srv, r, st);
         std::string err = rc != MTR_OK ? g_err : std::string();
         lk.lock();
         r->prep_rc = rc;
@@ -2875,13 +2878,13 @@ std::vector<ServerReq *> policy_order(const mtr_server *srv)
 }
 
 // Queues for preparation the requests the policy takes next, for the slots
-// the active ones leave free and one more, while the HBM budget allows
-// (their bytes count as in use from here on).
+// the active ones leave free and one more per worker, while the HBM budget
+// allows (their bytes count as in use from here on).
 void plan_prep(mtr_server *srv)
 {
     if (!srv->prefetch) return;
     const std::vector<ServerReq *> order = policy_order(srv);
-    const int want = std::max(srv->max_active - (int)srv->active.size(), 0) + 1;
+    const int want = std::max(srv->max_active - (int)srv->active.size(), 0) + (int)srv->workers.size();
     std::lock_guard<std::mutex> lk(srv->mu);
     for (int q = 0; q < (int)order.size() && q < want; q++) {
         ServerReq *r = order[q];
@@ -2891,7 +2894,7 @@ void plan_prep(mtr_server *srv)
         r->prep = PREP_QUEUED;
         srv->prep_q.push_back(r);
     }
-    if (!srv->prep_q.empty()) srv->cv_work.notify_one();
+    if (!srv->prep_q.empty()) srv->cv_work.notify_all();
 }
 
 // The admitted request's prepared context: waits for the worker if it is
@@ -3023,22 +3026,20 @@ int mtr_server_create(int device, int precision, size_t hbm_budget, int max_acti
     } catch (...) {  // no thread to spare: retire inside the steps
     }
     if (const char *pf = std::getenv("MTR_SERVER_PREFETCH")) srv->prefetch = std::atoi(pf) != 0;
-    if (srv->prefetch) {
-        if (hipStreamCreateWithFlags(&srv->prep_stream, hipStreamNonBlocking) != hipSuccess) {
-            (void)hipMemPoolDestroy(srv->pool);
-            (void)hipStreamDestroy(srv->out_stream);
-            mtr_ctx_destroy(eng);
-            delete srv;
-            return fail(MTR_ERR_HIP, "server preparation stream");
-        }
+    // as many workers as threads can be had, up to kPrepWorkers; with none,
+    // requests are prepared inside the ticks
+    for (int w = 0; srv->prefetch && w < kPrepWorkers; w++) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) break;
         try {
-            srv->worker = std::thread(prep_worker, srv);
-        } catch (...) {  // no thread to spare: prepare inside the ticks
-            (void)hipStreamDestroy(srv->prep_stream);
-            srv->prep_stream = nullptr;
-            srv->prefetch = false;
+            srv->workers.emplace_back(prep_worker, srv, st);
+        } catch (...) {
+            (void)hipStreamDestroy(st);
+            break;
         }
+        srv->prep_streams.push_back(st);
     }
+    if (srv->workers.empty()) srv->prefetch = false;
     *out = srv;
     return MTR_OK;
 }
@@ -3047,13 +3048,13 @@ int mtr_server_destroy(mtr_server *srv)
 {
     if (!srv) return MTR_OK;
     (void)hipSetDevice(srv->device);
-    if (srv->worker.joinable()) {
+    if (!srv->workers.empty()) {
         {
             std::lock_guard<std::mutex> lk(srv->mu);
             srv->quit = true;
         }
         srv->cv_work.notify_all();
-        srv->worker.join();
+        for (auto &w : srv->workers) w.join();
     }
     if (srv->retirer.joinable()) {
         {
@@ -3066,10 +3067,11 @@ int mtr_server_destroy(mtr_server *srv)
     (void)hipStreamSynchronize(srv->eng->stream);
     for (auto &kv : srv->reqs) release(srv, kv.second.get());
     // the frees just queued on the server's streams, then the pool
-    for (hipStream_t st : {srv->eng->stream, srv->out_stream, srv->prep_stream})
-        if (st) (void)hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(srv->eng->stream);
+    (void)hipStreamSynchronize(srv->out_stream);
+    for (hipStream_t st : srv->prep_streams) (void)hipStreamSynchronize(st);
     if (srv->pool) (void)hipMemPoolDestroy(srv->pool);
-    if (srv->prep_stream) (void)hipStreamDestroy(srv->prep_stream);
+    for (hipStream_t st : srv->prep_streams) (void)hipStreamDestroy(st);
     if (srv->out_stream) (void)hipStreamDestroy(srv->out_stream);
     if (srv->h_stage) (void)hipHostFree(srv->h_stage);
     if (srv->d_tab) (void)hipFree(srv->d_tab);
