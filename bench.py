@@ -10,15 +10,18 @@ anything is timed.
 
 A timed step is one MACE step: one replay of the captured HIP graph that
 holds 10 SIRT iterations of all four agents (each kernel kind launched once
-per iteration over its agents) and the fused consensus (z, Mann, mean, next
-z, FP64 consensus difference).  The initial model (BPT of every agent and
+per iteration over its agents; the column-plan agents at 45/135 deg on one
+stream and the others beside them on a second, so kernels of both kinds
+share the CUs) and the fused consensus (z, Mann, mean, next z, FP64
+consensus difference).  The initial model (BPT of every agent and
 their sum) is timed on its own, after a first initialisation that tunes the
 tiles, so
 
     value = reconstructions/s = 1 / (t_init + M * t_step).
 
 Per-kernel times come from HIP events recorded on the library's own stream
-around every launch of one further, host-issued MACE step.  The roofline is
+around every launch of one further, host-issued MACE step, which runs every
+kernel alone (timing keeps the kinds on one stream).  The roofline is
 taken for the kernel with the largest share of that step: FP32 FLOP (8 per
 voxel-view update) against the FP32 vector peak, with its HBM fraction on
 algorithmic bytes and the rocprof traffic per launch beside it.

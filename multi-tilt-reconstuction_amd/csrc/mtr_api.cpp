@@ -404,6 +404,11 @@ struct mtr_ctx {
     // ordered on the context's stream, so freeing it waits for no other
     // work on the device (hipFree does); other contexts use hipMalloc.
     hipMemPool_t pool = nullptr;
+    // A second stream for the SIRT iterations of the agents that are not
+    // column-plan ones, forked from and joined to `stream` by events.
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool kind_streams = true;
     int precision = MTR_PREC_FP32;
     mtr_volume_size vol{};
     bool vol_set = false;
@@ -491,19 +496,20 @@ hipEvent_t take_event(mtr_ctx *ctx)
 struct Timed {
     mtr_ctx *ctx;
     int kernel;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mtr_ctx *c, int k, bool on = true) : ctx(c), kernel(k)
+    Timed(mtr_ctx *c, int k, bool on = true, hipStream_t s = nullptr) : ctx(c), kernel(k), st(s ? s : c->stream)
     {
         if (ctx->timing && on) {
             a = take_event(ctx);
             b = take_event(ctx);
-            if (a && b) (void)hipEventRecord(a, ctx->stream);
+            if (a && b) (void)hipEventRecord(a, st);
         }
     }
     ~Timed()
     {
         if (a && b) {
-            (void)hipEventRecord(b, ctx->stream);
+            (void)hipEventRecord(b, st);
             ctx->pending.push_back({kernel, a, b});
         }
     }
@@ -705,20 +711,24 @@ struct Batch {
     AgentDev *d_tab = nullptr;
 };
 
+// Column-plan kinds sort after every other kind (a suffix of a sorted
+// batch: the SIRT loop runs them on a second stream).
 int fp_kind_key(const Agent &a)
 {
-    return a.col.on ? 100 + a.col.rw + (a.col_pfx ? 1000 : 0) + (a.col_win > 0 ? 2000 : 0)
+    return a.col.on ? 10100 + a.col.rw + (a.col_pfx ? 1000 : 0) + (a.col_win > 0 ? 2000 : 0)
                     : (a.fp_nc ? 10 + 8 * a.fp_nc + a.fp_g + (a.fp_seg > 1 ? 500 : 0) : (a.lanes_k ? 1 : 0));
 }
 int bp_kind_key(const Agent &a)
 {
-    return a.col.on ? 100 + a.col.rw + (a.col_cs ? 1000 : 0)
+    return a.col.on ? 10100 + a.col.rw + (a.col_cs ? 1000 : 0)
                     : (a.bpm.sep ? 10 + 4 * a.bpm.W + (a.bpm.rw == 32 ? 1 : (a.bpm.rw == 16 ? 2 : 3)) : 0);
 }
 
-// FP + residual of entries [first, first + count).
-int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
+// FP + residual of entries [first, first + count), on stream st (the
+// engine's stream when null).
+int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count, hipStream_t st = nullptr)
 {
+    if (!st) st = eng->stream;
     int s = first;
     const int end = first + count;
     while (s < end) {
@@ -749,38 +759,40 @@ int fp_batch(mtr_ctx *eng, const Batch &B, int first, int count)
                 }
             }
         }
-        Timed t(eng, MTR_K_FP);
+        Timed t(eng, MTR_K_FP, true, st);
         if (a0.col.on) {
             {
-                Timed tk(eng, MTR_K_FP_COLUMN);
+                Timed tk(eng, MTR_K_FP_COLUMN, true, st);
                 if (pfx)
-                    HIP_TRY(launch_fp_pfx(eng->precision, B.d_tab + s, e - s, mb, lds, false, win, eng->stream));
-                else HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, eng->stream));
+                    HIP_TRY(launch_fp_pfx(eng->precision, B.d_tab + s, e - s, mb, lds, false, win, st));
+                else HIP_TRY(launch_fp_col(eng->precision, B.d_tab + s, e - s, mb, nu, false, st));
             }
-            Timed tk(eng, MTR_K_FP_COLUMN_RES);
-            HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, eng->stream));
+            Timed tk(eng, MTR_K_FP_COLUMN_RES, true, st);
+            HIP_TRY(launch_fp_col_res(eng->precision, B.d_tab + s, e - s, mr, 0, nullptr, st));
         } else if (a0.fp_nc) {
             {
-                Timed tk(eng, MTR_K_FP_STREAM);
-                HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, eng->stream));
+                Timed tk(eng, MTR_K_FP_STREAM, true, st);
+                HIP_TRY(launch_fp_stream(eng->precision, B.d_tab + s, e - s, mb, a0.fp_nc, a0.fp_g, st));
             }
             // the kind key keeps segmented and whole-ray agents apart
             if (a0.fp_seg > 1) {
-                Timed tk(eng, MTR_K_FP_STREAM_RES);
-                HIP_TRY(launch_fp_seg_res(eng->precision, B.d_tab + s, e - s, mr, eng->stream));
+                Timed tk(eng, MTR_K_FP_STREAM_RES, true, st);
+                HIP_TRY(launch_fp_seg_res(eng->precision, B.d_tab + s, e - s, mr, st));
             }
         } else {
-            Timed tk(eng, MTR_K_FP_GENERAL);
-            HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, eng->stream));
+            Timed tk(eng, MTR_K_FP_GENERAL, true, st);
+            HIP_TRY(launch_fp(eng->precision, B.d_tab + s, e - s, mb, 0, a0.lanes_k, nullptr, nullptr, st));
         }
         s = e;
     }
     return MTR_OK;
 }
 
-// BP (bpt: initial model) of entries [first, first + count).
-int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float lambda)
+// BP (bpt: initial model) of entries [first, first + count), on stream st
+// (the engine's stream when null).
+int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float lambda, hipStream_t st = nullptr)
 {
+    if (!st) st = eng->stream;
     int s = first;
     const int end = first + count;
     while (s < end) {
@@ -795,19 +807,19 @@ int bp_batch(mtr_ctx *eng, const Batch &B, int first, int count, bool bpt, float
             mb = std::max(mb, cm.on ? (a0.col_cs ? bp_cs_blocks(d, 16) : bp_col_blocks(d, cm.rw))
                                     : (m.sep ? bp_sep_blocks(d, m.W, m.P1, m.P2, m.rw) : bp_blocks(d, eng->bp_tz)));
         }
-        Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP);
-        Timed tk(eng, cm.on ? MTR_K_BP_COLUMN : (m.sep ? MTR_K_BP_SEPARABLE : MTR_K_BP_GENERAL), !bpt);
+        Timed t(eng, bpt ? MTR_K_BPT : MTR_K_BP, true, st);
+        Timed tk(eng, cm.on ? MTR_K_BP_COLUMN : (m.sep ? MTR_K_BP_SEPARABLE : MTR_K_BP_GENERAL), !bpt, st);
         if (cm.on) {
             if (a0.col_cs)
-                HIP_TRY(launch_bp_cs(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, eng->stream));
+                HIP_TRY(launch_bp_cs(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, st));
             else
-                HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, eng->stream));
+                HIP_TRY(launch_bp_col(eng->precision, B.d_tab + s, e - s, mb, bpt ? 1 : 0, lambda, cm.rw, st));
         } else if (m.sep) {
             HIP_TRY(launch_bp_sep(eng->precision, B.d_tab + s, e - s, mb, bpt, lambda, m.W, m.P1, m.P2, m.rw,
-                                  eng->stream));
+                                  st));
         } else {
             HIP_TRY(launch_bp(eng->precision, B.d_tab + s, e - s, mb, eng->bp_tz, bpt, lambda, nullptr, eng->lds_cap,
-                              eng->stream));
+                              st));
         }
         s = e;
     }
@@ -1081,6 +1093,28 @@ int sirt_fast(mtr_ctx *ctx, const std::vector<int> &ids, int iters, float lambda
         // the whole context: each kernel kind once per iteration
         const Batch F = ctx_sorted(ctx, ctx->fp_order, ctx->d_ags_fp);
         const Batch Bk = ctx_sorted(ctx, ctx->bp_order, ctx->d_ags_bp);
+        // The column-plan agents (a suffix of both kind orders) iterate on
+        // the context's stream and the others on the second one: their
+        // kernels are bound by different units (VALU and LDS against the
+        // vector-memory path), and blocks of both kinds sharing the CUs
+        // overlap.  Every agent's iterations touch only its own buffers.
+        // Per-kernel timing keeps one stream, so each kernel is timed alone.
+        int fs = 0, bs = 0;
+        while (fs < K && !F.e[fs].ag->col.on) fs++;
+        while (bs < K && !Bk.e[bs].ag->col.on) bs++;
+        if (ctx->aux && ctx->kind_streams && !ctx->timing && fs > 0 && fs < K && fs == bs) {
+            HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+            HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev_fork, 0));
+            for (int it = 0; it < iters; it++) {
+                MTR_TRY(fp_batch(ctx, F, fs, K - fs));
+                MTR_TRY(bp_batch(ctx, Bk, bs, K - bs, false, lambda));
+                MTR_TRY(fp_batch(ctx, F, 0, fs, ctx->aux));
+                MTR_TRY(bp_batch(ctx, Bk, 0, bs, false, lambda, ctx->aux));
+            }
+            HIP_TRY(hipEventRecord(ctx->ev_join, ctx->aux));
+            HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+            return MTR_OK;
+        }
         for (int it = 0; it < iters; it++) {
             MTR_TRY(fp_batch(ctx, F, 0, K));
             MTR_TRY(bp_batch(ctx, Bk, 0, K, false, lambda));
@@ -1690,6 +1724,12 @@ int mtr_ctx_create(int device, mtr_ctx **out)
         delete ctx;
         return fail(MTR_ERR_HIP, "stream creation failed");
     }
+    if (hipStreamCreate(&ctx->aux) != hipSuccess || hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        mtr_ctx_destroy(ctx);
+        return fail(MTR_ERR_HIP, "stream creation failed");
+    }
+    if (const char *ks = std::getenv("MTR_KIND_STREAMS")) ctx->kind_streams = std::atoi(ks) != 0;
     if (const int rc = alloc_tables(ctx); rc != MTR_OK) {
         mtr_ctx_destroy(ctx);
         return rc;
@@ -1723,6 +1763,12 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
+    if (ctx->aux) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->stream && ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MTR_OK;

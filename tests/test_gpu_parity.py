@@ -320,6 +320,34 @@ def test_kind_sorted_batches_match_per_agent_calls(mtr):
     assert np.array_equal(fused, stepped)
 
 
+def test_kind_streams_give_identical_results(mtr, monkeypatch):
+    # the column-plan agents iterate on the context's stream and the others
+    # on a second one (MTR_KIND_STREAMS=1, the default) or all on one: each
+    # agent's iterations touch only its own buffers, so the solve, its
+    # consensus difference and every agent's state are the same bit for bit,
+    # replayed from a graph or issued from the host (timing keeps one stream)
+    pb = problem(mtr, [0.0, 45.0, 90.0, 135.0], X=40, Y=36, Z=16, views=9, seed=23)
+    out = {}
+    for ks in ("1", "0"):
+        monkeypatch.setenv("MTR_KIND_STREAMS", ks)
+        say("kind streams", ks)
+        with load_ctx(mtr, pb) as c:
+            g = c.mace_solve(3, 3, 0.5, 0.5)
+            d2 = c.consensus_diff2()
+            w = [c.get_volume(mtr.VOL_W, a) for a in range(pb.K)]
+            c.use_graphs(False)
+            e = c.mace_solve(2, 3, 0.5, 0.5)
+            c.timing(True)
+            t = c.mace_solve(2, 3, 0.5, 0.5)
+            c.timing(False)
+        out[ks] = (g, d2, w, e, t)
+    g1, d1, w1, e1, t1 = out["1"]
+    g0, d0, w0, e0, t0 = out["0"]
+    assert np.array_equal(g1, g0) and d1 == d0
+    assert all(np.array_equal(x, y) for x, y in zip(w1, w0))
+    assert np.array_equal(e1, e0) and np.array_equal(t1, t0) and np.array_equal(e1, t1)
+
+
 def test_elementwise_bit_exact(mtr, orc):
     pb = problem(mtr, [0.0, 90.0], X=16, Y=12, Z=6, views=5)
     rng = np.random.default_rng(12)
