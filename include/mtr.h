@@ -293,7 +293,10 @@ enum {
     MTR_K_FP_STREAM_RES = 14,/* Z-segmented streaming FP: sums + residual  */
     MTR_K_COUNT = 15
 };
-/* Enable (1) or disable (0) per-launch event timing. */
+/* Enable (1) or disable (0) per-launch event timing.  While it is on, the
+ * SIRT iterations keep every kernel on the context's stream (by default
+ * the column-plan agents and the others run side by side on two), so each
+ * kernel is timed alone; results are the same either way. */
 int mtr_timing_enable(mtr_ctx *ctx, int on);
 int mtr_timing_reset(mtr_ctx *ctx);
 /* Launch count and summed event time (ms) of one kernel class. */

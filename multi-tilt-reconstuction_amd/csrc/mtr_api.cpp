@@ -3280,8 +3280,10 @@ int mtr_server_step(mtr_server *srv, int *completed)
         r->state = REQ_ACTIVE;
         srv->active.push_back(r);
     }
-    // batch: every agent of every active request with steps left, most SIRT
-    // iterations first, then by relaxation and kernel
+    // batch: every agent of every active request with steps left, the
+    // column-plan agents after the others (the two lanes of the tick, on two
+    // streams), each lane most SIRT iterations first, then by relaxation and
+    // kernel
     Batch B;
     std::vector<const ServerReq *> owner;
     for (ServerReq *r : srv->active)
@@ -3293,6 +3295,7 @@ int mtr_server_step(mtr_server *srv, int *completed)
     std::vector<int> order(B.e.size());
     for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (B.e[x].ag->col.on != B.e[y].ag->col.on) return B.e[y].ag->col.on;
         if (owner[x]->S != owner[y]->S) return owner[x]->S > owner[y]->S;
         if (owner[x]->lambda != owner[y]->lambda) return owner[x]->lambda < owner[y]->lambda;
         const int fx = fp_kind_key(*B.e[x].ag), fy = fp_kind_key(*B.e[y].ag);
@@ -3326,18 +3329,39 @@ int mtr_server_step(mtr_server *srv, int *completed)
         HIP_TRY(hipMemcpyAsync(srv->d_tab, srv->h_tab.data(), sizeof(AgentDev) * n, hipMemcpyHostToDevice,
                                srv->eng->stream));
         B.d_tab = srv->d_tab;
-        const int maxS = owner[0]->S;
-        for (int it = 0; it < maxS; it++) {
-            int m = 0;
-            while (m < n && owner[m]->S > it) m++;
-            MTR_TRY(fp_batch(srv->eng, B, 0, m));
-            int s0 = 0;
-            while (s0 < m) {
-                int e0 = s0 + 1;
-                while (e0 < m && owner[e0]->lambda == owner[s0]->lambda) e0++;
-                MTR_TRY(bp_batch(srv->eng, B, s0, e0 - s0, false, owner[s0]->lambda));
-                s0 = e0;
+        // lanes [0, nb) and [nb, n): the others on the engine's second
+        // stream, the column-plan agents on its own (timing keeps one)
+        int nb = 0;
+        while (nb < n && !B.e[nb].ag->col.on) nb++;
+        mtr_ctx *eng = srv->eng;
+        const bool two = eng->aux && eng->kind_streams && !eng->timing && nb > 0 && nb < n;
+        if (two) {
+            HIP_TRY(hipEventRecord(eng->ev_fork, eng->stream));
+            HIP_TRY(hipStreamWaitEvent(eng->aux, eng->ev_fork, 0));
+        }
+        auto lane = [&](int l0, int l1, hipStream_t st) -> int {
+            for (int it = 0; l1 > l0 && it < owner[l0]->S; it++) {
+                int m = l0;
+                while (m < l1 && owner[m]->S > it) m++;
+                MTR_TRY(fp_batch(eng, B, l0, m - l0, st));
+                int s0 = l0;
+                while (s0 < m) {
+                    int e0 = s0 + 1;
+                    while (e0 < m && owner[e0]->lambda == owner[s0]->lambda) e0++;
+                    MTR_TRY(bp_batch(eng, B, s0, e0 - s0, false, owner[s0]->lambda, st));
+                    s0 = e0;
+                }
             }
+            return MTR_OK;
+        };
+        if (two) {
+            MTR_TRY(lane(nb, n, eng->stream));
+            MTR_TRY(lane(0, nb, eng->aux));
+            HIP_TRY(hipEventRecord(eng->ev_join, eng->aux));
+            HIP_TRY(hipStreamWaitEvent(eng->stream, eng->ev_join, 0));
+        } else {
+            MTR_TRY(lane(0, nb, eng->stream));
+            MTR_TRY(lane(nb, n, eng->stream));
         }
     }
     // consensus of every active request (MultiAxis.cpp:40-71)
