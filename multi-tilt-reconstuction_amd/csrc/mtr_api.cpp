@@ -3052,19 +3052,20 @@ int mtr_server_create(int device, int precision, size_t hbm_budget, int max_acti
     {
         // freed request memory stays in the pool (up to a quarter of the
         // budget) for the next requests rather than going back to the
-        // driver at each synchronization
+        // driver at each synchronization; without a pool (a device that
+        // has none) requests use hipMalloc and hipFree
         hipMemPoolProps pp = {};
         pp.allocType = hipMemAllocationTypePinned;
         pp.location.type = hipMemLocationTypeDevice;
         pp.location.id = device;
         uint64_t keep = hbm_budget / 4;
-        if (hipMemPoolCreate(&srv->pool, &pp) != hipSuccess ||
-            hipMemPoolSetAttribute(srv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
-            if (srv->pool) (void)hipMemPoolDestroy(srv->pool);
-            (void)hipStreamDestroy(srv->out_stream);
-            mtr_ctx_destroy(eng);
-            delete srv;
-            return fail(MTR_ERR_HIP, "server memory pool");
+        if (hipMemPoolCreate(&srv->pool, &pp) != hipSuccess) {
+            srv->pool = nullptr;
+            (void)hipGetLastError();
+        } else if (hipMemPoolSetAttribute(srv->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            (void)hipMemPoolDestroy(srv->pool);
+            srv->pool = nullptr;
+            (void)hipGetLastError();
         }
     }
     try {
