@@ -646,7 +646,7 @@ def main() -> int:
         "ms_per_step": t_step * 1e3,
         "ms_per_step_host_issued": t_step_eager * 1e3,
         # the step's projector and consensus kernels timed one at a time (the per-kernel
-        # pass; layout permutes, about 3.4 ms per step, not included): the graph step
+        # pass; the layout permutes not included): the graph step
         # overlaps kernels of the two kinds
         "projector_consensus_ms_per_step_alone": step_kernel_ms,
         "init_ms": t_init * 1e3,
