@@ -382,6 +382,7 @@ struct Agent {
     float *d_bpc = nullptr;    // column plan: BP normaliser per voxel
     bool col_cs = false;       // column plan: back-projection on strip tiles (k_bp_cs)
     ColGroup *d_col_grp = nullptr;
+    int2 *d_cs_win = nullptr;  // strip-tiled BP: footprint window per block and view
     int ngrp = 0;
     float *d_c = nullptr;
     void *d_r = nullptr;
@@ -563,7 +564,10 @@ size_t agent_bytes(int precision, const mtr_volume_size &vol, const mtr_projecti
     const size_t nfp = (size_t)((prj.X + 63) / 64 + 16) * ((prj.Y + 3) / 4 + 16) * prj.AngN;
     return (sizeof(ViewCoef) + sizeof(FpPlan) + sizeof(BpSepView)) * prj.AngN + 2 * 4 * P + 4 * P + es * P +
            2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj.Y + 2) + 8 * nfp +
-           sizeof(ColGroup) * (cols / 64 + (size_t)prj.Y + 2) + fp_seg_bytes(P, prj, vol.Z);
+           sizeof(ColGroup) * (cols / 64 + (size_t)prj.Y + 2) + fp_seg_bytes(P, prj, vol.Z) +
+           // strip-tiled BP windows: groups x blocks of 128 planes (one more
+           // for a box off the 16-plane grid) x views
+           sizeof(int2) * (cols / 64 + (size_t)prj.Y + 2) * ((size_t)vol.Z / 128 + 2) * prj.AngN;
 }
 
 // Host-to-device copy of a host buffer that is freed after the call,
@@ -619,6 +623,7 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.pn = ag.d_pn;
     d.bpc = ag.d_bpc;
     d.col_grp = ag.d_col_grp;
+    d.cs_win = ag.d_cs_win;
     d.ngrp = ag.ngrp;
     d.seg_part = ag.d_seg;
     d.nseg = ag.fp_seg;
@@ -1249,7 +1254,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
                         (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
-                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_seg, (void *)g.d_seg_ord})
+                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_cs_win, (void *)g.d_seg, (void *)g.d_seg_ord})
             dev_free(ctx, q);
         ctx->agents.pop_back();
         return rc;
@@ -1455,6 +1460,11 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
                 ALLOC(ag.d_col_grp, sizeof(ColGroup) * grp.size());
                 if (upload_now(ag.d_col_grp, grp.data(), sizeof(ColGroup) * grp.size(), ctx->stream) != hipSuccess)
                     return undo(fail(MTR_ERR_HIP, "column group upload failed"));
+                const std::vector<int2> win =
+                    bp_cs_windows(grp.data(), ag.ngrp, xc, A, ctx->vol.Zstart, ctx->vol.Zend);
+                ALLOC(ag.d_cs_win, sizeof(int2) * win.size());
+                if (upload_now(ag.d_cs_win, win.data(), sizeof(int2) * win.size(), ctx->stream) != hipSuccess)
+                    return undo(fail(MTR_ERR_HIP, "back-projection window upload failed"));
             }
         }
         if (upload_now(ag.d_col_xy, xy.data(), sizeof(int2) * xy.size(), ctx->stream) != hipSuccess ||
@@ -1751,7 +1761,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
-                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_seg, (void *)ag.d_seg_ord})
+                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_cs_win, (void *)ag.d_seg, (void *)ag.d_seg_ord})
             dev_free(ctx, q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,

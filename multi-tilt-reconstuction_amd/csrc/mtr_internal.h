@@ -95,6 +95,9 @@ struct AgentDev {
     float *bpc;            // back-projection normaliser C of every voxel (working layout)
     const ColGroup *col_grp;  // strip-tiled back-projection: column groups
     int ngrp;
+    // strip-tiled back-projection: detector-column window (first, last) of
+    // each block's footprint in each view, [(group * nzg + zgroup) * A + view]
+    const int2 *cs_win;
     // Streaming forward projection in Z segments of seg_len planes (nseg >
     // 1): per-segment partial ray sums [nseg][A][...] (strides pk/pi),
     // added and turned into the residual by a second kernel.

@@ -54,6 +54,9 @@ int bp_col_blocks(const AgentDev &ag, int rw);
 int bp_cs_blocks(const AgentDev &ag, int rw);
 int bp_cs_planes();
 int cs_max_width();
+// The window table of ag.cs_win for groups grp[0 .. ngrp) of a box whose
+// planes are [Zs, Ze), views xc[4 A].
+std::vector<int2> bp_cs_windows(const ColGroup *grp, int ngrp, const double *xc, int A, int Zs, int Ze);
 hipError_t launch_bp_cs(int prec, const AgentDev *d_ags, int nag, int max_blocks, int mode, float lambda,
                         hipStream_t st);
 int bp_col_waves();

@@ -2241,10 +2241,43 @@ constexpr int kCsMaxW = 512;
 
 int cs_max_width() { return kCsMaxW - 2; }
 
+// Footprint of a block in one view: the detector columns over the group's
+// box and the block's planes Zb0 .. Zb1, one column of margin on each side,
+// from the reference's FP64 coordinate (the window only bounds what is
+// staged: every tap of the block falls inside it).
+static void cs_window(const double *uc, const ColGroup &g, int Zb0, int Zb1, int &fi0, int &fi1)
+{
+    const double ux = std::fmin(uc[1] * (double)g.xmin, uc[1] * (double)g.xmax);
+    const double uy = std::fmin(uc[2] * (double)g.ymin, uc[2] * (double)g.ymax);
+    const double uz = std::fmin(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
+    const double vx = std::fmax(uc[1] * (double)g.xmin, uc[1] * (double)g.xmax);
+    const double vy = std::fmax(uc[2] * (double)g.ymin, uc[2] * (double)g.ymax);
+    const double vz = std::fmax(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
+    fi0 = (int)std::floor(uc[0] + ux + uy + uz - 1.0) - 1;
+    fi1 = (int)std::floor(uc[0] + vx + vy + vz - 1.0) + 1;
+}
+
+std::vector<int2> bp_cs_windows(const ColGroup *grp, int ngrp, const double *xc, int A, int Zs, int Ze)
+{
+    const int zal = col_run_base(Zs, 16), ntz = col_runs(Zs, Ze, 16), nzg = (ntz + kCsWaves - 1) / kCsWaves;
+    std::vector<int2> w((size_t)ngrp * nzg * A);
+    for (int gi = 0; gi < ngrp; gi++)
+        for (int zg = 0; zg < nzg; zg++) {
+            const int Zb0 = zal + zg * kCsWaves * 16, Zb1 = Zb0 + kCsWaves * 16 - 1;
+            for (int a = 0; a < A; a++) {
+                int f0, f1;
+                cs_window(xc + 4 * a, grp[gi], Zb0, Zb1, f0, f1);
+                w[((size_t)gi * nzg + zg) * A + a] = make_int2(f0, f1);
+            }
+        }
+    return w;
+}
+
 template <class T, int RW, int MODE>
 __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restrict__ ags, float lambda)
 {
     constexpr bool BPT = MODE == 1, NORM = MODE == 2;
+    static_assert(RW == 16, "the host's window table (bp_cs_windows) is for runs of 16 planes");
     constexpr int kEpiF4 = kCsWaves * 64 * 17 / 4;
     constexpr int kLds = 2 * kCsMaxW > kEpiF4 ? 2 * kCsMaxW : kEpiF4;
     __shared__ float4 lds4[kLds];
@@ -2259,12 +2292,11 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
     const int gi = b / nzg, zg = b - gi * nzg;
     const __attribute__((address_space(4))) int *gp =
         (const __attribute__((address_space(4))) int *)(ag.col_grp + gi);
-    const int gj0 = gp[0], gn = gp[1], xmin = gp[2], xmax = gp[3], ymin = gp[4], ymax = gp[5], k0 = gp[6];
+    const int gj0 = gp[0], gn = gp[1], k0 = gp[6];
     const int lane = threadIdx.x & 63, wid = wave_id();
     const int zal = col_run_base(ag.Zs, RW);
     const int bz = zg * kCsWaves + wid;
     const int Z0 = zal + bz * RW;
-    const int Zb0 = zal + zg * kCsWaves * RW, Zb1 = Zb0 + kCsWaves * RW - 1;
     const int Z32 = Z0 >= 0 ? (Z0 / 32) * 32 : -(((-Z0) + 31) / 32) * 32;
     const int zlo = max(ag.Zs - Z0, 0);
     const int ew = min(RW, ag.Ze - Z0) - 1;
@@ -2289,17 +2321,12 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         if constexpr (!NORM) S[t] = 0.0f;
         if constexpr (NORM) D[t] = 0.0f;
     }
-    // footprint of view a: detector columns over the group's box and the
-    // block's planes, one column of margin on each side
-    auto window = [&](const double (&uc)[4], int &fi0, int &fi1) {
-        const double ux = fmin(uc[1] * (double)xmin, uc[1] * (double)xmax);
-        const double uy = fmin(uc[2] * (double)ymin, uc[2] * (double)ymax);
-        const double uz = fmin(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
-        const double vx = fmax(uc[1] * (double)xmin, uc[1] * (double)xmax);
-        const double vy = fmax(uc[2] * (double)ymin, uc[2] * (double)ymax);
-        const double vz = fmax(uc[3] * (double)Zb0, uc[3] * (double)Zb1);
-        fi0 = (int)floor(uc[0] + ux + uy + uz - 1.0) - 1;
-        fi1 = (int)floor(uc[0] + vx + vy + vz - 1.0) + 1;
+    // footprint of view a (the host's cs_window), through the scalar cache
+    const __attribute__((address_space(4))) int *wtab =
+        (const __attribute__((address_space(4))) int *)(ag.cs_win + ((int64_t)gi * nzg + zg) * ag.A);
+    auto window = [&](int a, int &fi0, int &fi1) {
+        fi0 = wtab[2 * a];
+        fi1 = wtab[2 * a + 1];
     };
     float4 pre;
     auto fetch = [&](int a, int fi0, int fi1) {
@@ -2318,7 +2345,7 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
     int fi0, fi1;
     double ucn[4];
     load_ucoef(ag.coef, 0, ucn);
-    window(ucn, fi0, fi1);
+    window(0, fi0, fi1);
     if constexpr (!NORM) {
         fetch(0, fi0, fi1);
         lds4[threadIdx.x] = pre;
@@ -2330,7 +2357,7 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         const double uc[4] = {ucn[0], ucn[1], ucn[2], ucn[3]};
         if (a + 1 < nview) {
             load_ucoef(ag.coef, a + 1, ucn);
-            window(ucn, fi0, fi1);
+            window(a + 1, fi0, fi1);
             if constexpr (!NORM) fetch(a + 1, fi0, fi1);
         }
         const double ub = fma(uc[3], (double)Z32, fma(uc[2], (double)Y, fma(uc[1], (double)X, uc[0]))) - 1.0;
