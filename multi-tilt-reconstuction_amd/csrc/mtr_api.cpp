@@ -383,6 +383,7 @@ struct Agent {
     bool col_cs = false;       // column plan: back-projection on strip tiles (k_bp_cs)
     ColGroup *d_col_grp = nullptr;
     int2 *d_cs_win = nullptr;  // strip-tiled BP: footprint window per block and view
+    int2 *d_sep_win = nullptr;  // separable BP: footprint window per tile and view
     int ngrp = 0;
     float *d_c = nullptr;
     void *d_r = nullptr;
@@ -566,8 +567,11 @@ size_t agent_bytes(int precision, const mtr_volume_size &vol, const mtr_projecti
            2 * 4 * P + 3 * es * N + 4 * N + 12 * cols + 4 * (size_t)(prj.Y + 2) + 8 * nfp +
            sizeof(ColGroup) * (cols / 64 + (size_t)prj.Y + 2) + fp_seg_bytes(P, prj, vol.Z) +
            // strip-tiled BP windows: groups x blocks of 128 planes (one more
-           // for a box off the 16-plane grid) x views
-           sizeof(int2) * (cols / 64 + (size_t)prj.Y + 2) * ((size_t)vol.Z / 128 + 2) * prj.AngN;
+           // for a box off the 16-plane grid) x views; separable BP windows:
+           // tiles of 64 x 8 lines x views
+           sizeof(int2) * (cols / 64 + (size_t)prj.Y + 2) * ((size_t)vol.Z / 128 + 2) * prj.AngN +
+           sizeof(int2) * ((size_t)std::max({vol.X, vol.Y, vol.Z}) / 64 + 1) *
+               ((size_t)std::max({vol.X, vol.Y, vol.Z}) / 8 + 1) * prj.AngN;
 }
 
 // Host-to-device copy of a host buffer that is freed after the call,
@@ -624,6 +628,7 @@ AgentDev make_dev(const mtr_ctx *ctx, const Agent &ag)
     d.bpc = ag.d_bpc;
     d.col_grp = ag.d_col_grp;
     d.cs_win = ag.d_cs_win;
+    d.sep_win = ag.d_sep_win;
     d.ngrp = ag.ngrp;
     d.seg_part = ag.d_seg;
     d.nseg = ag.fp_seg;
@@ -1254,7 +1259,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         for (void *q : {(void *)g.d_coef, (void *)g.d_plan, (void *)g.d_bsv, g.p_owned ? (void *)g.d_p : nullptr, (void *)g.d_c,
                         g.d_r, g.d_x, g.d_w, g.d_tmp, (void *)g.d_partial, (void *)g.d_scalar, (void *)g.d_p_std,
                         (void *)g.d_col_xy, (void *)g.d_col_w, (void *)g.d_strip, (void *)g.d_pm, (void *)g.d_pn,
-                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_cs_win, (void *)g.d_seg, (void *)g.d_seg_ord})
+                        (void *)g.d_bpc, (void *)g.d_col_grp, (void *)g.d_cs_win, (void *)g.d_sep_win, (void *)g.d_seg, (void *)g.d_seg_ord})
             dev_free(ctx, q);
         ctx->agents.pop_back();
         return rc;
@@ -1343,6 +1348,12 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
         if (hipMemcpyAsync(ag.d_bsv, bsv.data(), sizeof(BpSepView) * A, hipMemcpyHostToDevice, ctx->stream) !=
             hipSuccess)
             return undo(fail(MTR_ERR_HIP, "geometry upload failed"));
+    }
+    if (!ag.col.on && !bsv.empty()) {
+        const std::vector<int2> sw = bp_sep_windows(bsv.data(), A, make_dev(ctx, ag), ag.bpm.P1, ag.bpm.P2);
+        ALLOC(ag.d_sep_win, sizeof(int2) * sw.size());
+        if (upload_now(ag.d_sep_win, sw.data(), sizeof(int2) * sw.size(), ctx->stream) != hipSuccess)
+            return undo(fail(MTR_ERR_HIP, "back-projection window upload failed"));
     }
     ALLOC(ag.d_c, sizeof(float) * P);
     ALLOC(ag.d_r, es * P);
@@ -1559,7 +1570,7 @@ int agent_add_common(mtr_ctx *ctx, const mtr_projection_size *prj, const double 
 
 // Temporary single-agent device state for the stateless operators (FP32).
 struct TmpAgent {
-    DevBuf coef, plan, bsv, x, s, c, r, tab;
+    DevBuf coef, plan, bsv, sepw, x, s, c, r, tab;
     AgentDev dev{};
     bool lanes_k = false;
     BpMode bpm;
@@ -1629,7 +1640,15 @@ int tmp_agent(mtr_ctx *ctx, const mtr_projection_size *prj, const mtr_volume_siz
     d.sx = 1;
     d.sy = vol->X;
     d.sz = (int64_t)vol->X * vol->Y;
+    std::vector<int2> sepw;
+    if (!bsv.empty()) {
+        sepw = bp_sep_windows(bsv.data(), A, d, t.bpm.P1, t.bpm.P2);
+        HIP_TRY(hipMalloc(&t.sepw.p, sizeof(int2) * sepw.size()));
+        HIP_TRY(hipMemcpyAsync(t.sepw.p, sepw.data(), sizeof(int2) * sepw.size(), hipMemcpyHostToDevice, ctx->stream));
+        d.sep_win = (const int2 *)t.sepw.p;
+    }
     HIP_TRY(hipMemcpyAsync(t.tab.p, &d, sizeof(AgentDev), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));  // the host tables are freed on return
     t.nfp = fp_blocks(d, t.lanes_k);
     return MTR_OK;
 }
@@ -1761,7 +1780,7 @@ int mtr_ctx_destroy(mtr_ctx *ctx)
         for (void *q : {(void *)ag.d_coef, (void *)ag.d_plan, (void *)ag.d_bsv, ag.p_owned ? ag.d_p : nullptr, (void *)ag.d_c, ag.d_r,
                         ag.d_x, ag.d_w, ag.d_tmp, (void *)ag.d_partial, (void *)ag.d_scalar, (void *)ag.d_p_std,
                         (void *)ag.d_col_xy, (void *)ag.d_col_w, (void *)ag.d_strip, (void *)ag.d_pm, (void *)ag.d_pn,
-                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_cs_win, (void *)ag.d_seg, (void *)ag.d_seg_ord})
+                        (void *)ag.d_bpc, (void *)ag.d_col_grp, (void *)ag.d_cs_win, (void *)ag.d_sep_win, (void *)ag.d_seg, (void *)ag.d_seg_ord})
             dev_free(ctx, q);
     }
     for (void *q : {ctx->d_wbar, (void *)ctx->d_ags, (void *)ctx->d_ags_fp, (void *)ctx->d_ags_bp,

@@ -98,6 +98,9 @@ struct AgentDev {
     // strip-tiled back-projection: detector-column window (first, last) of
     // each block's footprint in each view, [(group * nzg + zgroup) * A + view]
     const int2 *cs_win;
+    // separable back-projection: the same per (P1, P2) tile and view,
+    // [(tile1 + nt1 * tile2) * A + view]
+    const int2 *sep_win;
     // Streaming forward projection in Z segments of seg_len planes (nseg >
     // 1): per-segment partial ray sums [nseg][A][...] (strides pk/pi),
     // added and turned into the residual by a second kernel.

@@ -1,6 +1,8 @@
 // Launchers of the HIP kernels (host-callable, defined in mtr_kernels.hip).
 #pragma once
 
+#include <vector>
+
 #include "mtr_internal.h"
 
 namespace mtr {
@@ -21,6 +23,9 @@ hipError_t launch_fp_seg_res(int prec, const AgentDev *d_ags, int nag, int block
 int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw);
 int bp_sep_pitch();
 int bp_sep_waves();
+// The window table of ag.sep_win for the views bsv[A] of an agent whose box
+// and detector width ag holds, tiles along P1 and P2.
+std::vector<int2> bp_sep_windows(const BpSepView *bsv, int A, const AgentDev &ag, int P1, int P2);
 hipError_t launch_bp_sep(int prec, const AgentDev *d_ags, int nag, int max_blocks, bool bpt, float lambda, int W,
                          int P1, int P2, int rw, hipStream_t st);
 

@@ -940,10 +940,13 @@ __global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__res
     // full runs: every staged row is on the detector (host-checked)
     const bool full_run = ew == RW - 1;
 
+    // the tile's footprint window in each view (the host's sep_window),
+    // through the scalar cache
+    const __attribute__((address_space(4))) int *wtab =
+        (const __attribute__((address_space(4))) int *)(ag.sep_win + (int64_t)(b1 + nt1 * b2) * ag.A);
     auto fetch = [&](const BpSepView &bv, int a, float2(&pv)[St::kPer], int &fi0, int &width) {
-        const double u0 = fma(bv.a1, (double)P10, fma(bv.a2, (double)P20, bv.c0));
-        fi0 = max((int)floor(u0 + bv.umin) - 1, -1);
-        const int fi1 = min((int)floor(u0 + bv.umax) + 2, nu);
+        fi0 = wtab[2 * a];
+        const int fi1 = wtab[2 * a + 1];
         width = fi1 - fi0 + 1;
         const int k0row = bv.k0 + bv.kstep * W0, kstep = bv.kstep;
         const uint32_t vbase = (uint32_t)(plane * a);
@@ -2894,6 +2897,33 @@ int bp_sep_blocks(const AgentDev &ag, int W, int P1, int P2, int rw)
 
 int bp_sep_pitch() { return kSepPitch; }
 int bp_sep_waves() { return kSepWaves; }
+
+// Footprint of a separable BP tile (64 x kSepWaves lines from P10, P20) in
+// one view: detector columns from the reference's FP64 coordinate at the
+// tile's corner and the view's u range over the tile, with the margins the
+// staging needs, clamped to [-1, nu].
+static void sep_window(const BpSepView &bv, int P10, int P20, int nu, int &fi0, int &fi1)
+{
+    const double u0 = std::fma(bv.a1, (double)P10, std::fma(bv.a2, (double)P20, bv.c0));
+    fi0 = std::max((int)std::floor(u0 + bv.umin) - 1, -1);
+    fi1 = std::min((int)std::floor(u0 + bv.umax) + 2, nu);
+}
+
+std::vector<int2> bp_sep_windows(const BpSepView *bsv, int A, const AgentDev &ag, int P1, int P2)
+{
+    auto ext = [&](int ax) { return ax == 0 ? ag.X : (ax == 1 ? ag.Y : ag.Z); };
+    auto start = [&](int ax) { return ax == 0 ? ag.Xs : (ax == 1 ? ag.Ys : ag.Zs); };
+    const int nt1 = (ext(P1) + 63) / 64, nt2 = (ext(P2) + kSepWaves - 1) / kSepWaves;
+    std::vector<int2> w((size_t)nt1 * nt2 * A);
+    for (int b2 = 0; b2 < nt2; b2++)
+        for (int b1 = 0; b1 < nt1; b1++)
+            for (int a = 0; a < A; a++) {
+                int f0, f1;
+                sep_window(bsv[a], start(P1) + b1 * 64, start(P2) + b2 * kSepWaves, ag.nu, f0, f1);
+                w[((size_t)b1 + (size_t)nt1 * b2) * A + a] = make_int2(f0, f1);
+            }
+    return w;
+}
 
 template <class T, int RW>
 static void launch_bp_sep_rw(dim3 grid, const AgentDev *d_ags, bool bpt, float lambda, BpSepArgs sa, hipStream_t st)
