@@ -2349,9 +2349,14 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
     double ucn[4];
     load_ucoef(ag.coef, 0, ucn);
     window(0, fi0, fi1);
+    // a wave whose 64 staging slots all lie past the footprint's width
+    // stages nothing (taps never read past it)
+    bool stage = wid * 64 <= fi1 - fi0;
     if constexpr (!NORM) {
-        fetch(0, fi0, fi1);
-        lds4[threadIdx.x] = pre;
+        if (stage) {
+            fetch(0, fi0, fi1);
+            lds4[threadIdx.x] = pre;
+        }
     }
     __syncthreads();
     const int nview = k0 < -1 ? 0 : ag.A;  // block-uniform
@@ -2361,7 +2366,9 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         if (a + 1 < nview) {
             load_ucoef(ag.coef, a + 1, ucn);
             window(a + 1, fi0, fi1);
-            if constexpr (!NORM) fetch(a + 1, fi0, fi1);
+            stage = wid * 64 <= fi1 - fi0;
+            if constexpr (!NORM)
+                if (stage) fetch(a + 1, fi0, fi1);
         }
         const double ub = fma(uc[3], (double)Z32, fma(uc[2], (double)Y, fma(uc[1], (double)X, uc[0]))) - 1.0;
         const double fb = floor(ub);
@@ -2398,7 +2405,7 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         if (RW == 16 && Z0 != Z32) taps(std::integral_constant<int, 16>{});
         else taps(std::integral_constant<int, 0>{});
         if constexpr (!NORM) {
-            if (a + 1 < nview) lds4[((a + 1) & 1) * kCsMaxW + threadIdx.x] = pre;
+            if (a + 1 < nview && stage) lds4[((a + 1) & 1) * kCsMaxW + threadIdx.x] = pre;
             __syncthreads();
         }
     }
