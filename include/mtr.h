@@ -331,8 +331,8 @@ int mtr_sync(mtr_ctx *ctx);
  * caller's work between steps overlaps the GPU's.  A request whose last
  * MACE step was launched is done after the next mtr_server_step;
  * mtr_server_result waits for its hand-over.  Request memory comes from a
- * stream-ordered pool of the server's, so a retirement waits for no other
- * device work.  A request's result does not depend on what else shares
+ * stream-ordered pool of the server's (plain allocations on a device
+ * without memory pools), so a retirement waits for no other device work.  A request's result does not depend on what else shares
  * its launches.
  * Times are seconds since mtr_server_create. */
 typedef struct mtr_server mtr_server;
