@@ -321,8 +321,8 @@ int mtr_sync(mtr_ctx *ctx);
  * max_active are active.  The workers prepare the requests the policy
  * takes next (device memory, geometry tables, uploads; their bytes count
  * as in use from then on) while ticks run, and admission forms the
- * BPT-sum initial model.  Server calls are for one host thread.  Each mtr_server_step first
- * completes the previous step (waits for its device work; the requests
+ * BPT-sum initial model.  Server calls are for one host thread.  Each
+ * mtr_server_step first completes the previous step (waits for its device work; the requests
  * that finished with it count as done and go to a second thread, the
  * retirer, which copies their w-bar to the host and frees their device
  * memory while the next steps run), then admits, then launches one MACE
@@ -332,8 +332,8 @@ int mtr_sync(mtr_ctx *ctx);
  * MACE step was launched is done after the next mtr_server_step;
  * mtr_server_result waits for its hand-over.  Request memory comes from a
  * stream-ordered pool of the server's (plain allocations on a device
- * without memory pools), so a retirement waits for no other device work.  A request's result does not depend on what else shares
- * its launches.
+ * without memory pools), so a retirement waits for no other device work.
+ * A request's result does not depend on what else shares its launches.
  * Times are seconds since mtr_server_create. */
 typedef struct mtr_server mtr_server;
 int mtr_server_create(int device, int precision, size_t hbm_budget, int max_active, mtr_server **out);
