@@ -473,8 +473,9 @@ def main() -> int:
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--serve-requests", type=int, default=64, help="serving stream length (0 = skip serving)")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0)
-    ap.add_argument("--max-active", type=int, default=8,
-                    help="concurrently active requests (8: lowest p99 on the seeded stream, 2-64 swept)")
+    ap.add_argument("--max-active", type=int, default=6,
+                    help="concurrently active requests (6: p50 a fifth below 8 slots at the same p99 on the seeded "
+                         "stream; 2-16 swept)")
     ap.add_argument("--admit", default="smallest", choices=["arrival", "smallest"],
                     help="serving admission order: arrival, or least work first with a bound on the wait "
                          "(on the seeded stream p50 about 3.7 s lower at the same p99)")
