@@ -874,8 +874,8 @@ __device__ __forceinline__ BpSepView load_bsv(const BpSepView *tab, int a)
 }
 
 template <class T, int RW, int PITCH, bool BPT, int VPS>
-__global__ __launch_bounds__(64 * kSepWaves) void k_bp_sep(const AgentDev *__restrict__ ags, float lambda,
-                                                          BpSepArgs sa)
+__global__ __launch_bounds__(64 * kSepWaves) __attribute__((amdgpu_waves_per_eu(6))) void k_bp_sep(
+    const AgentDev *__restrict__ ags, float lambda, BpSepArgs sa)
 {
     constexpr int NT = 64 * kSepWaves;
     // LDS holds (r[k][i], r[k][i+1] - r[k][i]) pairs so that the interior
@@ -2888,8 +2888,10 @@ hipError_t launch_fp_stream(int prec, const AgentDev *d_ags, int nag, int max_bl
 }
 
 // odd (in float2 units) so that a column of 32 rows written by 32 lanes
-// spreads over all LDS banks
-constexpr int kSepPitch = 81;
+// spreads over all LDS banks; 79 so that a footprint of 32 rows stages in
+// five slots per thread (32 x 79 <= 5 x 512), which with the register cap
+// below fits three blocks per CU without spills
+constexpr int kSepPitch = 79;
 // views staged per barrier: two for runs of up to 16 planes (the four
 // footprints then still fit three blocks per CU in LDS), one for 32
 template <int RW>
