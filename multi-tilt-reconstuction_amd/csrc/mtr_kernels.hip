@@ -2374,17 +2374,18 @@ __global__ __launch_bounds__(64 * kCsWaves) void k_bp_cs(const AgentDev *__restr
         const double fb = floor(ub);
         const float f0 = (float)(ub - fb), step = (float)uc[3];
         const int ib = (int)fb - cfi0;
-        const int lmax = min(cfi1 - cfi0, kCsMaxW - 1);
         const float4 *stg = lds4 + (a & 1) * kCsMaxW + ib;
-        const float glo = (float)(-ib), ghi = (float)(lmax - ib) + 0.5f;
         auto taps = [&](auto off_c) {
             constexpr int OFF = decltype(off_c)::value;
             if constexpr (!NORM) {
 #pragma unroll
                 for (int t = 0; t < RW; t++) {
                     const float g = fmaf((float)(OFF + t), step, f0);
-                    float du;
-                    const int li = tap_clamped(g, glo, ghi, du);
+                    // no clamp: the window covers the block's box over all its
+                    // planes with a column of margin, far more than the FP32
+                    // steps of g can drift from the FP64 corners
+                    const float du = __builtin_amdgcn_fractf(g);
+                    const int li = cvt_flr(g);
                     const float4 q = stg[li];
                     S[t] = fmaf(wk0, fmaf(du, q.y, q.x), S[t]);
                     S[t] = fmaf(wk1, fmaf(du, q.w, q.z), S[t]);
